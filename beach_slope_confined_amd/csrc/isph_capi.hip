@@ -1,0 +1,1083 @@
+// isph_capi.hip -- host side of the C ABI declared in include/isph_capi.h.
+//
+// A context owns every device buffer (allocated once at isph_create, sized
+// for cap_particles, like memory.f90:10-147 sizes the reference's cells)
+// and one HIP stream of default priority.  Each entry point enqueues its
+// kernels on that stream and synchronises before returning; isph_step()
+// runs whole steps and synchronises only inside the PPE solve (to poll the
+// device-side convergence flag) and at the end.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/isph_capi.h"
+#include "isph_kernels.h"
+
+using namespace isph;
+
+namespace {
+
+struct DevBuf {
+    StateView v;
+};
+
+struct EventPair {
+    hipEvent_t a, b;
+};
+
+struct Profiler {
+    bool on = false;
+    std::vector<EventPair> pool[4];
+    std::vector<char> real[4];      // per recorded launch: did real work
+    size_t used[4] = {0, 0, 0, 0};
+    int64_t launches[4] = {0, 0, 0, 0};
+    double total_ms[4] = {0, 0, 0, 0};
+};
+
+}  // namespace
+
+struct isph_ctx {
+    isph_params prm;
+    int32_t cap = 0, cap_soil = 0, n = 0, ns = 0;
+    GridInfo grid;
+    hipStream_t stream = nullptr;
+    StateView st[2];
+    int cur = 0;
+    double* L = nullptr;
+    double* divr = nullptr;
+    int *key = nullptr, *key_s = nullptr, *rank = nullptr, *perm = nullptr;
+    int *bincnt = nullptr, *scanbuf = nullptr;
+    int *nbr = nullptr, *cnt = nullptr;
+    bool have_nbr = false;
+    double *val64 = nullptr, *rhs = nullptr;
+    float* val32 = nullptr;
+    int* dpos = nullptr;
+    bool assembled = false;
+    double* v64[12] = {};
+    float* v32[10] = {};
+    double* part = nullptr;
+    uint32_t* counters = nullptr;
+    SolverScalars* sc = nullptr;
+    SolverScalars* sc_host = nullptr;  // pinned
+    SimScalars* ss = nullptr;
+    unsigned long long* maxbuf = nullptr;
+    uint32_t* err = nullptr;
+    double2* spos = nullptr;
+    double *smass = nullptr, *srho = nullptr;
+    int* sstart = nullptr;
+    int iters = 0, outer = 0;
+    double relres = 0.0;
+    Profiler prof;
+    std::string msg;
+};
+
+namespace {
+
+#define HIPCHK(ctx, expr)                                                             \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            if (ctx) (ctx)->msg = std::string(#expr) + ": " + hipGetErrorString(e_);  \
+            return ISPH_E_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+int fail(isph_ctx* c, int code, const char* m) {
+    c->msg = m;
+    return code;
+}
+
+inline int nblk(int64_t n, int b = BLOCK) { return (int)((n + b - 1) / b); }
+inline int nblk8(int64_t n) { return ((nblk(n) + 7) / 8) * 8; }
+
+GridInfo make_grid(const isph_params& p) {
+    GridInfo g;
+    g.xmin = p.xmin;
+    g.ymin = p.ymin;
+    g.cs = p.cell_size;
+    g.nx = (int)std::floor((p.xmax - p.xmin) / p.cell_size) + 3;
+    g.ny = (int)std::floor((p.ymax - p.ymin) / p.cell_size) + 3;
+    g.nbins = 2LL * g.nx * g.ny;
+    return g;
+}
+
+int check_params(isph_ctx* c, const isph_params* p) {
+    if (!(p->h > 0.0) || !(p->cell_size >= 2.0 * p->h) || !(p->xmax > p->xmin) ||
+        !(p->ymax > p->ymin) || p->max_nbr <= 0)
+        return fail(c, ISPH_E_ARG, "invalid geometry parameters");
+    GridInfo g = make_grid(*p);
+    if ((double)g.nx * (double)g.ny * 2.0 > 2.0e9) return fail(c, ISPH_E_ARG, "cell grid too large");
+    return ISPH_OK;
+}
+
+PorousParams porous_params(const isph_params& p) {
+    PorousParams pp;
+    pp.por = p.por;
+    pp.alpha = p.alpha;
+    pp.beta = p.beta;
+    pp.d50 = p.d50;
+    pp.nu = p.nu;
+    pp.drag_law = p.drag_law;
+    pp.visc_law = p.visc_law;
+    return pp;
+}
+
+template <class T>
+int dalloc(isph_ctx* c, T** p, size_t count) {
+    HIPCHK(c, hipMalloc((void**)p, sizeof(T) * (count ? count : 1)));
+    HIPCHK(c, hipMemsetAsync(*p, 0, sizeof(T) * (count ? count : 1), c->stream));
+    return ISPH_OK;
+}
+
+int alloc_state(isph_ctx* c, StateView& s, size_t cap) {
+    int rc = 0;
+    rc |= dalloc(c, &s.pos, cap);
+    rc |= dalloc(c, &s.vel, cap);
+    rc |= dalloc(c, &s.sp, cap);
+    rc |= dalloc(c, &s.sv, cap);
+    double** d[] = {&s.mass, &s.rho, &s.oden, &s.p, &s.por, &s.R, &s.nut, &s.tke, &s.varts, &s.velm};
+    for (auto pp : d) rc |= dalloc(c, pp, cap);
+    rc |= dalloc(c, &s.tid, cap);
+    rc |= dalloc(c, &s.flags, cap);
+    rc |= dalloc(c, &s.pid, cap);
+    return rc ? ISPH_E_HIP : ISPH_OK;
+}
+
+void free_state(StateView& s) {
+    void* p[] = {s.pos, s.vel, s.sp, s.sv, s.mass, s.rho, s.oden, s.p, s.por, s.R, s.nut, s.tke,
+                 s.varts, s.velm, s.tid, s.flags, s.pid};
+    for (auto q : p)
+        if (q) (void)hipFree(q);
+}
+
+// read and clear the device error word; map to a status
+int take_errors(isph_ctx* c) {
+    uint32_t e = 0;
+    HIPCHK(c, hipMemcpyAsync(&e, c->err, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!e) return ISPH_OK;
+    HIPCHK(c, hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->stream));
+    if (e & ERR_NAN) return fail(c, ISPH_E_NAN, "non-finite velocity");
+    if (e & ERR_SINGULAR) return fail(c, ISPH_E_SINGULAR, "PPE row with zero or non-finite diagonal");
+    if (e & ERR_CAPACITY) return fail(c, ISPH_E_CAPACITY, "neighbour list capacity exceeded");
+    if (e & ERR_DOMAIN) return fail(c, ISPH_E_DOMAIN, "particle outside the cell grid");
+    if (e & ERR_NOCONV) return fail(c, ISPH_E_NOCONV, "PPE solve did not converge");
+    return fail(c, ISPH_E_HIP, "unknown device error");
+}
+
+int sync_check(isph_ctx* c) {
+    HIPCHK(c, hipGetLastError());
+    return take_errors(c);
+}
+
+// exclusive scan of `count` ints in place
+int scan(isph_ctx* c, int* data, int64_t count) {
+    int nb = nblk(count, SCAN_BLOCK);
+    k_scan_blocks<<<nb, SCAN_BLOCK, 0, c->stream>>>((int)count, data, c->scanbuf);
+    k_scan_sums<<<1, SCAN_BLOCK, 0, c->stream>>>(nb, c->scanbuf);
+    k_scan_add<<<nb, SCAN_BLOCK, 0, c->stream>>>((int)count, data, c->scanbuf);
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+// stable counting sort by key = 2*cell + fluid bit (deleted: key nbins)
+int sort_particles(isph_ctx* c, bool with_sink) {
+    const int n = c->n;
+    const int64_t nb = c->grid.nbins;
+    StateView& s = c->st[c->cur];
+    StateView& d = c->st[c->cur ^ 1];
+    HIPCHK(c, hipMemsetAsync(c->bincnt, 0, sizeof(int) * (nb + 2), c->stream));
+    if (n > 0)
+        k_bin_keys<<<nblk(n), BLOCK, 0, c->stream>>>(
+            n, c->grid, s.pos, s.tid, s.flags, with_sink && c->prm.sink, c->prm.xlcutoff,
+            c->prm.ytcutoff, c->key, c->bincnt, c->rank, c->err);
+    int rc = scan(c, c->bincnt, nb + 2);
+    if (rc) return rc;
+    if (n > 0) {
+        k_bin_scatter<<<nblk(n), BLOCK, 0, c->stream>>>(n, c->key, c->rank, c->bincnt, c->perm);
+        k_bin_fixup<<<nblk(nb + 1), BLOCK, 0, c->stream>>>((int)(nb + 1), c->bincnt, c->perm);
+    }
+    int nnew = n;
+    if (with_sink && c->prm.sink) {
+        HIPCHK(c, hipMemcpyAsync(&nnew, c->bincnt + nb, sizeof(int), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (nnew > 0)
+        k_gather_state<<<nblk(nnew), BLOCK, 0, c->stream>>>(nnew, c->perm, s, d, c->key, c->key_s);
+    HIPCHK(c, hipGetLastError());
+    c->cur ^= 1;
+    c->n = nnew;
+    c->have_nbr = false;
+    c->assembled = false;
+    return ISPH_OK;
+}
+
+KernelConsts kconsts(const isph_ctx* c) { return make_kc(c->prm); }
+
+// ---- stage launchers (enqueue only) ----------------------------------------
+int run_projection(isph_ctx* c) {
+    if (c->n) k_projection<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->st[c->cur], c->ss);
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+int run_neighbour(isph_ctx* c) {
+    const isph_params& p = c->prm;
+    if (c->n)
+        k_neighbour<<<nblk(c->n), BLOCK, 0, c->stream>>>(
+            c->n, c->grid, kconsts(c), p.lamfs * p.maxdivr, p.max_nbr, (size_t)c->cap,
+            c->st[c->cur], c->key_s, c->bincnt, c->nbr, c->cnt, c->L, c->divr, c->err);
+    HIPCHK(c, hipGetLastError());
+    c->have_nbr = true;
+    c->assembled = false;
+    return ISPH_OK;
+}
+
+int run_effpor(isph_ctx* c) {
+    if (c->n)
+        k_effpor<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->grid, kconsts(c),
+                                                       porous_params(c->prm), c->st[c->cur],
+                                                       c->spos, c->smass, c->srho, c->sstart,
+                                                       c->err);
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+int run_eddyvis(isph_ctx* c) {
+    if (!c->have_nbr) return fail(c, ISPH_E_STATE, "eddyvis before neighbour");
+    if (c->n)
+        k_eddyvis<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, kconsts(c), c->prm.dl, (size_t)c->cap,
+                                                        c->st[c->cur], c->nbr, c->cnt, c->L);
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+int run_int_vel(isph_ctx* c) {
+    if (!c->have_nbr) return fail(c, ISPH_E_STATE, "int_vel before neighbour");
+    if (c->n) {
+        k_varts<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, kconsts(c), c->prm.dl, (size_t)c->cap,
+                                                      c->st[c->cur], c->nbr, c->cnt, c->L, c->ss);
+        k_vstar<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, kconsts(c), porous_params(c->prm),
+                                                      c->prm.lam, c->prm.g, (size_t)c->cap,
+                                                      c->st[c->cur], c->nbr, c->cnt, c->L, c->ss);
+    }
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+int run_assemble(isph_ctx* c) {
+    if (!c->have_nbr) return fail(c, ISPH_E_STATE, "assembly before neighbour");
+    const isph_params& p = c->prm;
+    if (c->n)
+        k_assemble<<<nblk(c->n), BLOCK, 0, c->stream>>>(
+            c->n, kconsts(c), p.lam, p.rho0 * p.rel_den, (size_t)c->cap, c->st[c->cur], c->nbr,
+            c->cnt, c->L, c->val64, p.precision == ISPH_PREC_MIXED ? c->val32 : nullptr, c->dpos,
+            c->rhs, c->ss, c->err);
+    HIPCHK(c, hipGetLastError());
+    c->assembled = true;
+    return ISPH_OK;
+}
+
+int run_comp_vel(isph_ctx* c) {
+    if (!c->have_nbr) return fail(c, ISPH_E_STATE, "comp_vel before neighbour");
+    if (c->n)
+        k_comp_vel<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, kconsts(c), (size_t)c->cap,
+                                                         c->st[c->cur], c->nbr, c->cnt, c->L,
+                                                         c->ss, c->err);
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+int run_comp_pos(isph_ctx* c) {
+    if (c->n) k_comp_pos<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->st[c->cur], c->ss);
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+int run_freesurf(isph_ctx* c) {
+    if (!c->have_nbr) return fail(c, ISPH_E_STATE, "freesurf before neighbour");
+    if (c->n)
+        k_freesurf<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->prm.lamfs * c->prm.maxdivr,
+                                                         (size_t)c->cap, c->st[c->cur], c->nbr,
+                                                         c->cnt, c->divr);
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+int run_shift(isph_ctx* c) {
+    if (!c->have_nbr) return fail(c, ISPH_E_STATE, "opt2_shift before neighbour");
+    if (!c->prm.shifting || !c->n) return ISPH_OK;
+    k_shift<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, kconsts(c), c->prm.csh,
+                                                  c->prm.maxshift * c->prm.dl, (size_t)c->cap,
+                                                  c->st[c->cur], c->nbr, c->cnt, c->L);
+    k_shift_apply<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->st[c->cur]);
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+int run_timestep(isph_ctx* c) {
+    HIPCHK(c, hipMemsetAsync(c->maxbuf, 0, 2 * sizeof(unsigned long long), c->stream));
+    if (c->n)
+        k_maxes<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->st[c->cur].velm, c->st[c->cur].nut,
+                                                      c->maxbuf);
+    const isph_params& p = c->prm;
+    k_timestep<<<1, 64, 0, c->stream>>>(c->maxbuf, c->ss, p.dl, p.sig1, p.mu, p.rho0,
+                                        p.adaptive_dt);
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+// ---- PPE solve -------------------------------------------------------------
+__global__ void k_reset_solver(SolverScalars* sc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sc->iters = 0;
+        sc->total = 0;
+        sc->done = 0;
+        sc->status = 0;
+        sc->first = 1;
+        sc->relres = 0.0;
+    }
+}
+
+__global__ void k_fill(int n, double* x, double v) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+int read_sc(isph_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync(c->sc_host, c->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISPH_OK;
+}
+
+void prof_record(isph_ctx* c, int k, bool start) {
+    Profiler& pr = c->prof;
+    if (!pr.on) return;
+    if (start) {
+        if (pr.used[k] == pr.pool[k].size()) {
+            EventPair e;
+            (void)hipEventCreate(&e.a);
+            (void)hipEventCreate(&e.b);
+            pr.pool[k].push_back(e);
+            pr.real[k].push_back(0);
+        }
+        (void)hipEventRecord(pr.pool[k][pr.used[k]].a, c->stream);
+    } else {
+        (void)hipEventRecord(pr.pool[k][pr.used[k]].b, c->stream);
+        pr.real[k][pr.used[k]] = 1;
+        pr.used[k]++;
+    }
+}
+
+// Of the `issued` launches of the last batch only the first `real_iters`
+// did work; the rest returned at once because the device had set `done`.
+void prof_mark(isph_ctx* c, int k0, int real_iters, int issued) {
+    Profiler& pr = c->prof;
+    if (!pr.on) return;
+    for (int k = k0; k < k0 + 2; ++k) {
+        size_t start = pr.used[k] - (size_t)issued;
+        for (size_t q = start + (size_t)std::max(real_iters, 0); q < pr.used[k]; ++q) pr.real[k][q] = 0;
+    }
+}
+
+void prof_collect(isph_ctx* c) {
+    Profiler& pr = c->prof;
+    if (!pr.on) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 4; ++k) {
+        for (size_t q = 0; q < pr.used[k]; ++q) {
+            if (!pr.real[k][q]) continue;
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, pr.pool[k][q].a, pr.pool[k][q].b) == hipSuccess) {
+                pr.total_ms[k] += ms;
+                pr.launches[k] += 1;
+            }
+        }
+        pr.used[k] = 0;
+    }
+}
+
+template <class T>
+SolveArgsT<T> targs(isph_ctx* c, const T* val, T* const* v, double tol) {
+    SolveArgsT<T> a;
+    a.n = c->n;
+    a.ld = (size_t)c->cap;
+    a.nbr = c->nbr;
+    a.cnt = c->cnt;
+    a.dpos = c->dpos;
+    a.val = val;
+    a.rs = v[0];
+    a.rhat = v[1];
+    a.si = v[6];
+    a.mm2 = v[7];
+    a.xs = v[8];
+    a.part = c->part;
+    a.counters = c->counters;
+    a.tol = tol;
+    a.maxit = c->prm.maxit;
+    a.inner_maxit = c->prm.inner_maxit;
+    return a;
+}
+
+SolveArgs sargs(isph_ctx* c) {
+    SolveArgs a;
+    a.n = c->n;
+    a.ld = (size_t)c->cap;
+    a.nbr = c->nbr;
+    a.cnt = c->cnt;
+    a.dpos = c->dpos;
+    a.val = c->val64;
+    a.b = c->rhs;
+    a.x = c->v64[8];
+    a.rs = c->v64[0];
+    a.rhat = c->v64[1];
+    a.part = c->part;
+    a.counters = c->counters;
+    a.tol = c->prm.tol;
+    a.maxit = c->prm.maxit;
+    return a;
+}
+
+// Run BiCGSTAB iterations in batches; the device sets sc->done, the host
+// polls it after each batch.  Vectors v: 0 rs, 1 rhat, 2/3 p ping-pong,
+// 4/5 mm1 ping-pong, 6 si, 7 mm2, 8 x.  prof slot k0 (SpMV kernels).
+template <class T>
+int iterate(isph_ctx* c, const SolveArgsT<T>& a, T* const* v, int inner, int k0) {
+    const int grid = nblk8(c->n);
+    int cidx = 0;
+    for (;;) {
+        int before = c->sc_host->iters;
+        int batch = 8;
+        if (c->sc_host->nb > 0 && c->sc_host->rr > 0) {
+            double rel = std::sqrt(c->sc_host->rr) / (inner ? std::sqrt(c->sc_host->rr0) : c->sc_host->nb);
+            batch = rel > 1e3 * a.tol ? 32 : (rel > 10.0 * a.tol ? 16 : 4);
+        }
+        for (int b = 0; b < batch; ++b) {
+            prof_record(c, k0, true);
+            k_bicg_1<T><<<grid, BLOCK, 0, c->stream>>>(a, c->sc, v[2 + cidx], v[4 + cidx],
+                                                        v[3 - cidx], v[5 - cidx]);
+            prof_record(c, k0, false);
+            prof_record(c, k0 + 1, true);
+            k_bicg_2<T><<<grid, BLOCK, 0, c->stream>>>(a, c->sc, v[5 - cidx]);
+            prof_record(c, k0 + 1, false);
+            k_bicg_3<T><<<grid, BLOCK, 0, c->stream>>>(a, c->sc, v[3 - cidx], inner);
+            cidx ^= 1;
+        }
+        HIPCHK(c, hipGetLastError());
+        int rc = read_sc(c);
+        if (rc) return rc;
+        prof_mark(c, k0, c->sc_host->iters - before, batch);
+        if (c->sc_host->done) return ISPH_OK;
+    }
+}
+
+int solve_fp64(isph_ctx* c) {
+    const int grid = nblk8(c->n);
+    SolveArgs a = sargs(c);
+    double* const* v = c->v64;
+    SolveArgsT<double> ta = targs<double>(c, c->val64, v, c->prm.tol);
+    k_reset_solver<<<1, 64, 0, c->stream>>>(c->sc);
+    k_bicg_init<<<grid, BLOCK, 0, c->stream>>>(a, c->sc);
+    HIPCHK(c, hipGetLastError());
+    int rc = read_sc(c);
+    if (rc) return rc;
+    for (;;) {
+        int d = c->sc_host->done;
+        if (d == DONE_ZERO_RHS) {
+            k_fill<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, a.x, 0.0);
+            return ISPH_OK;
+        }
+        if (d == DONE_CONVERGED) return ISPH_OK;
+        if (d == DONE_FAILED) return fail(c, ISPH_E_NOCONV, "PPE solve did not converge");
+        if (d == DONE_RECURRENCE) {
+            k_bicg_true<<<grid, BLOCK, 0, c->stream>>>(a, c->sc);
+            HIPCHK(c, hipGetLastError());
+            if ((rc = read_sc(c))) return rc;
+            continue;
+        }
+        if ((rc = iterate<double>(c, ta, v, 0, 0))) return rc;
+    }
+}
+
+int solve_mixed(isph_ctx* c) {
+    const int grid = nblk8(c->n);
+    SolveArgs a = sargs(c);
+    float* const* v = c->v32;
+    SolveArgsT<float> fa = targs<float>(c, c->val32, v, c->prm.inner_tol);
+    double* r64 = c->v64[9];
+    k_reset_solver<<<1, 64, 0, c->stream>>>(c->sc);
+    c->outer = 0;
+    int rc;
+    for (;;) {
+        k_mixed_resid<<<grid, BLOCK, 0, c->stream>>>(a, c->sc, r64);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = read_sc(c))) return rc;
+        int d = c->sc_host->done;
+        if (d == DONE_ZERO_RHS) {
+            k_fill<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, a.x, 0.0);
+            return ISPH_OK;
+        }
+        if (d == DONE_CONVERGED) return ISPH_OK;
+        if (c->outer >= c->prm.max_outer || c->sc_host->total >= c->prm.maxit)
+            return fail(c, ISPH_E_NOCONV, "mixed-precision PPE solve did not converge");
+        c->outer++;
+        k_mixed_start<<<grid, BLOCK, 0, c->stream>>>(fa, c->sc, r64);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = read_sc(c))) return rc;
+        if ((rc = iterate<float>(c, fa, v, 1, 2))) return rc;
+        k_mixed_update<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, a.x, v[8], c->sc);
+    }
+}
+
+int run_solve(isph_ctx* c) {
+    if (!c->assembled) return fail(c, ISPH_E_STATE, "solve before assembly");
+    if (c->prm.precision == ISPH_PREC_MIXED && !c->val32)
+        return fail(c, ISPH_E_STATE, "mixed precision needs an assembly with precision=MIXED");
+    StateView& s = c->st[c->cur];
+    double* x = c->v64[8];
+    if (c->n == 0) return ISPH_OK;
+    HIPCHK(c, hipMemcpyAsync(x, s.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    int rc = c->prm.precision == ISPH_PREC_MIXED ? solve_mixed(c) : solve_fp64(c);
+    if (rc == ISPH_OK) rc = read_sc(c);
+    c->iters = c->sc_host->total;
+    c->relres = c->sc_host->relres;
+    if (c->prm.precision != ISPH_PREC_MIXED) c->outer = 0;
+    if (rc != ISPH_OK) return rc;
+    // pressure scatter (isph.f90:465-483): rows are particles
+    HIPCHK(c, hipMemcpyAsync(s.p, x, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    return ISPH_OK;
+}
+
+int run_step(isph_ctx* c) {
+    int rc;
+    k_advance_time<<<1, 64, 0, c->stream>>>(c->ss);
+    if ((rc = run_projection(c))) return rc;
+    if ((rc = sort_particles(c, false))) return rc;
+    if ((rc = run_neighbour(c))) return rc;
+    if ((rc = run_effpor(c))) return rc;
+    if ((rc = run_eddyvis(c))) return rc;
+    if ((rc = run_int_vel(c))) return rc;
+    if ((rc = run_assemble(c))) return rc;
+    if ((rc = run_solve(c))) return rc;
+    if ((rc = take_errors(c))) return rc;
+    if ((rc = run_comp_vel(c))) return rc;
+    if ((rc = run_comp_pos(c))) return rc;
+    if ((rc = sort_particles(c, true))) return rc;
+    if ((rc = run_neighbour(c))) return rc;
+    if ((rc = run_freesurf(c))) return rc;
+    if ((rc = run_effpor(c))) return rc;
+    if ((rc = run_shift(c))) return rc;
+    if ((rc = run_timestep(c))) return rc;
+    return ISPH_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+int isph_create(const isph_params* prm, int32_t cap, int32_t cap_soil, isph_ctx** out) {
+    if (!prm || !out || cap <= 0 || cap_soil < 0) return ISPH_E_ARG;
+    isph_ctx* c = new isph_ctx();
+    *out = nullptr;
+    int rc = check_params(c, prm);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    c->prm = *prm;
+    c->cap = cap;
+    c->cap_soil = cap_soil;
+    c->grid = make_grid(*prm);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return ISPH_E_HIP;
+    }
+    const size_t N = (size_t)cap, M = (size_t)prm->max_nbr;
+    const int64_t nb = c->grid.nbins;
+    rc |= alloc_state(c, c->st[0], N);
+    rc |= alloc_state(c, c->st[1], N);
+    rc |= dalloc(c, &c->L, 4 * N);
+    rc |= dalloc(c, &c->divr, N);
+    rc |= dalloc(c, &c->key, N);
+    rc |= dalloc(c, &c->key_s, N);
+    rc |= dalloc(c, &c->rank, N);
+    rc |= dalloc(c, &c->perm, N);
+    rc |= dalloc(c, &c->bincnt, (size_t)nb + 2);
+    rc |= dalloc(c, &c->scanbuf, (size_t)nblk(nb + 2, SCAN_BLOCK) + 1);
+    rc |= dalloc(c, &c->nbr, N * M);
+    rc |= dalloc(c, &c->cnt, N);
+    rc |= dalloc(c, &c->val64, N * M);
+    rc |= dalloc(c, &c->val32, N * M);
+    rc |= dalloc(c, &c->dpos, N);
+    rc |= dalloc(c, &c->rhs, N);
+    for (int k = 0; k < 10; ++k) rc |= dalloc(c, &c->v64[k], N);
+    for (int k = 0; k < 9; ++k) rc |= dalloc(c, &c->v32[k], N);
+    rc |= dalloc(c, &c->part, 4 * (size_t)nblk8(cap));
+    rc |= dalloc(c, &c->counters, 16);
+    rc |= dalloc(c, &c->sc, 1);
+    rc |= dalloc(c, &c->ss, 1);
+    rc |= dalloc(c, &c->maxbuf, 2);
+    rc |= dalloc(c, &c->err, 1);
+    rc |= dalloc(c, &c->spos, (size_t)std::max(cap_soil, 1));
+    rc |= dalloc(c, &c->smass, (size_t)std::max(cap_soil, 1));
+    rc |= dalloc(c, &c->srho, (size_t)std::max(cap_soil, 1));
+    rc |= dalloc(c, &c->sstart, (size_t)(nb / 2) + 1);
+    if (hipHostMalloc((void**)&c->sc_host, sizeof(SolverScalars), hipHostMallocDefault) != hipSuccess)
+        rc = ISPH_E_HIP;
+    if (rc) {
+        isph_destroy(c);
+        return ISPH_E_HIP;
+    }
+    memset(c->sc_host, 0, sizeof(SolverScalars));
+    SimScalars ss0 = {0.0, 0.0, prm->dt, 0.0};
+    if (hipMemcpyAsync(c->ss, &ss0, sizeof(ss0), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        isph_destroy(c);
+        return ISPH_E_HIP;
+    }
+    *out = c;
+    return ISPH_OK;
+}
+
+int isph_destroy(isph_ctx* c) {
+    if (!c) return ISPH_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_state(c->st[0]);
+    free_state(c->st[1]);
+    void* p[] = {c->L, c->divr, c->key, c->key_s, c->rank, c->perm, c->bincnt, c->scanbuf,
+                 c->nbr, c->cnt, c->val64, c->val32, c->dpos, c->rhs, c->part, c->counters,
+                 c->sc, c->ss, c->maxbuf, c->err, c->spos, c->smass, c->srho, c->sstart};
+    for (auto q : p)
+        if (q) (void)hipFree(q);
+    for (int k = 0; k < 12; ++k)
+        if (c->v64[k]) (void)hipFree(c->v64[k]);
+    for (int k = 0; k < 10; ++k)
+        if (c->v32[k]) (void)hipFree(c->v32[k]);
+    if (c->sc_host) (void)hipHostFree(c->sc_host);
+    for (int k = 0; k < 4; ++k)
+        for (auto& e : c->prof.pool[k]) {
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ISPH_OK;
+}
+
+const char* isph_last_error(const isph_ctx* c) { return c ? c->msg.c_str() : "null context"; }
+
+int isph_set_params(isph_ctx* c, const isph_params* prm) {
+    if (!c || !prm) return ISPH_E_ARG;
+    int rc = check_params(c, prm);
+    if (rc) return rc;
+    GridInfo g = make_grid(*prm);
+    if (g.nx != c->grid.nx || g.ny != c->grid.ny || prm->max_nbr != c->prm.max_nbr)
+        return fail(c, ISPH_E_ARG, "grid and max_nbr are fixed at create time");
+    c->prm = *prm;
+    c->grid = g;
+    c->assembled = false;
+    return ISPH_OK;
+}
+
+int isph_get_params(const isph_ctx* c, isph_params* prm) {
+    if (!c || !prm) return ISPH_E_ARG;
+    *prm = c->prm;
+    return ISPH_OK;
+}
+
+int isph_set_particles(isph_ctx* c, int32_t n, const double* x, const double* y,
+                       const double* vx, const double* vy, const double* mass,
+                       const double* density, const double* oden, const double* pressure,
+                       const int32_t* tid, const int32_t* flags, const int32_t* pid) {
+    if (!c) return ISPH_E_ARG;
+    if (n < 0 || n > c->cap) return fail(c, ISPH_E_ARG, "n exceeds capacity");
+    if (n > 0 && (!x || !y || !vx || !vy || !mass || !density || !oden || !pressure || !tid || !pid))
+        return fail(c, ISPH_E_ARG, "null particle array");
+    StateView& s = c->st[c->cur];
+    std::vector<double2> a(n), b(n), z(n, make_double2(0.0, 0.0));
+    std::vector<double> velm(n), one(n, 1.0), zero(n, 0.0);
+    std::vector<int> fl(n, 0);
+    for (int i = 0; i < n; ++i) {
+        a[i] = make_double2(x[i], y[i]);
+        b[i] = make_double2(vx[i], vy[i]);
+        velm[i] = std::sqrt(vx[i] * vx[i] + vy[i] * vy[i]);
+        if (flags) fl[i] = flags[i];
+    }
+    const size_t nd = sizeof(double) * n, ni = sizeof(int) * n, n2 = sizeof(double2) * n;
+    auto H2D = hipMemcpyHostToDevice;
+    HIPCHK(c, hipMemcpyAsync(s.pos, a.data(), n2, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.vel, b.data(), n2, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.sp, z.data(), n2, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.sv, z.data(), n2, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.mass, mass, nd, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.rho, density, nd, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.oden, oden, nd, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.p, pressure, nd, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.por, one.data(), nd, H2D, c->stream));
+    double* zs[] = {s.R, s.nut, s.tke, s.varts};
+    for (auto q : zs) HIPCHK(c, hipMemcpyAsync(q, zero.data(), nd, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.velm, velm.data(), nd, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.tid, tid, ni, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.flags, fl.data(), ni, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.pid, pid, ni, H2D, c->stream));
+    std::vector<double> Lh(4 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        Lh[4 * i] = 1.0; Lh[4 * i + 1] = 0.0; Lh[4 * i + 2] = 0.0; Lh[4 * i + 3] = 1.0;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->L, Lh.data(), 4 * nd, H2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->divr, zero.data(), nd, H2D, c->stream));
+    SimScalars ss0 = {0.0, 0.0, c->prm.dt, 0.0};
+    HIPCHK(c, hipMemcpyAsync(c->ss, &ss0, sizeof(ss0), H2D, c->stream));
+    c->n = n;
+    int rc = sort_particles(c, false);
+    if (rc) return rc;
+    return sync_check(c);
+}
+
+int isph_get_particles(isph_ctx* c, int32_t* n, double* x, double* y, double* vx, double* vy,
+                       double* mass, double* density, double* oden, double* pressure,
+                       int32_t* tid, int32_t* flags, int32_t* pid) {
+    if (!c) return ISPH_E_ARG;
+    if (n) *n = c->n;
+    const int m = c->n;
+    StateView& s = c->st[c->cur];
+    const size_t nd = sizeof(double) * m, ni = sizeof(int) * m;
+    auto D2H = hipMemcpyDeviceToHost;
+    std::vector<double2> a;
+    if (x || y) {
+        a.resize(m);
+        HIPCHK(c, hipMemcpyAsync(a.data(), s.pos, sizeof(double2) * m, D2H, c->stream));
+    }
+    std::vector<double2> b;
+    if (vx || vy) {
+        b.resize(m);
+        HIPCHK(c, hipMemcpyAsync(b.data(), s.vel, sizeof(double2) * m, D2H, c->stream));
+    }
+    if (mass) HIPCHK(c, hipMemcpyAsync(mass, s.mass, nd, D2H, c->stream));
+    if (density) HIPCHK(c, hipMemcpyAsync(density, s.rho, nd, D2H, c->stream));
+    if (oden) HIPCHK(c, hipMemcpyAsync(oden, s.oden, nd, D2H, c->stream));
+    if (pressure) HIPCHK(c, hipMemcpyAsync(pressure, s.p, nd, D2H, c->stream));
+    if (tid) HIPCHK(c, hipMemcpyAsync(tid, s.tid, ni, D2H, c->stream));
+    if (flags) HIPCHK(c, hipMemcpyAsync(flags, s.flags, ni, D2H, c->stream));
+    if (pid) HIPCHK(c, hipMemcpyAsync(pid, s.pid, ni, D2H, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < m; ++i) {
+        if (x) x[i] = a[i].x;
+        if (y) y[i] = a[i].y;
+        if (vx) vx[i] = b[i].x;
+        if (vy) vy[i] = b[i].y;
+    }
+    return ISPH_OK;
+}
+
+int isph_set_soil(isph_ctx* c, int32_t ns, const double* x, const double* y, const double* mass,
+                  const double* density) {
+    if (!c) return ISPH_E_ARG;
+    if (ns < 0 || ns > c->cap_soil) return fail(c, ISPH_E_ARG, "ns exceeds soil capacity");
+    const GridInfo& g = c->grid;
+    const int64_t nc = g.nbins / 2;
+    std::vector<int> cnt(nc + 1, 0), k(ns);
+    bool oob = false;
+    for (int i = 0; i < ns; ++i) {
+        double fx = std::floor((x[i] - g.xmin) / g.cs), fy = std::floor((y[i] - g.ymin) / g.cs);
+        int cx, cy;
+        if (!(fx >= 0.0)) { cx = 1; oob = true; }
+        else if (fx > (double)(g.nx - 3)) { cx = g.nx - 2; oob = true; }
+        else cx = (int)fx + 1;
+        if (!(fy >= 0.0)) { cy = 1; oob = true; }
+        else if (fy > (double)(g.ny - 3)) { cy = g.ny - 2; oob = true; }
+        else cy = (int)fy + 1;
+        k[i] = cx * g.ny + cy;
+        cnt[k[i] + 1]++;
+    }
+    for (int64_t b = 0; b < nc; ++b) cnt[b + 1] += cnt[b];
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    std::vector<double2> sp(ns);
+    std::vector<double> sm(ns), sr(ns);
+    for (int i = 0; i < ns; ++i) {
+        int d = pos[k[i]]++;
+        sp[d] = make_double2(x[i], y[i]);
+        sm[d] = mass[i];
+        sr[d] = density[i];
+    }
+    auto H2D = hipMemcpyHostToDevice;
+    if (ns) {
+        HIPCHK(c, hipMemcpyAsync(c->spos, sp.data(), sizeof(double2) * ns, H2D, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->smass, sm.data(), sizeof(double) * ns, H2D, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->srho, sr.data(), sizeof(double) * ns, H2D, c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->sstart, cnt.data(), sizeof(int) * (nc + 1), H2D, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ns = ns;
+    if (oob) return fail(c, ISPH_E_DOMAIN, "soil particle outside the cell grid");
+    return ISPH_OK;
+}
+
+static double* field_ptr(isph_ctx* c, int32_t f, int* width, bool* pair, int* comp) {
+    StateView& s = c->st[c->cur];
+    *width = 1;
+    *pair = false;
+    *comp = 0;
+    switch (f) {
+        case ISPH_FIELD_XS: *pair = true; *comp = 0; return (double*)s.sp;
+        case ISPH_FIELD_YS: *pair = true; *comp = 1; return (double*)s.sp;
+        case ISPH_FIELD_VXS: *pair = true; *comp = 0; return (double*)s.sv;
+        case ISPH_FIELD_VYS: *pair = true; *comp = 1; return (double*)s.sv;
+        case ISPH_FIELD_POROSITY: return s.por;
+        case ISPH_FIELD_RESIST: return s.R;
+        case ISPH_FIELD_COFF: *width = 4; return c->L;
+        case ISPH_FIELD_DIVR: return c->divr;
+        case ISPH_FIELD_NUT: return s.nut;
+        case ISPH_FIELD_TKE: return s.tke;
+        case ISPH_FIELD_VARTS: return s.varts;
+        case ISPH_FIELD_VEL: return s.velm;
+        case ISPH_FIELD_RHS: return c->rhs;
+        default: return nullptr;
+    }
+}
+
+int isph_get_field(isph_ctx* c, int32_t f, double* out) {
+    if (!c || !out) return ISPH_E_ARG;
+    int w, comp;
+    bool pair;
+    double* p = field_ptr(c, f, &w, &pair, &comp);
+    if (!p) return fail(c, ISPH_E_ARG, "unknown field");
+    const int m = c->n;
+    if (pair) {
+        std::vector<double> t(2 * (size_t)m);
+        HIPCHK(c, hipMemcpyAsync(t.data(), p, sizeof(double) * 2 * m, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < m; ++i) out[i] = t[2 * (size_t)i + comp];
+        return ISPH_OK;
+    }
+    HIPCHK(c, hipMemcpyAsync(out, p, sizeof(double) * w * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISPH_OK;
+}
+
+int isph_set_field(isph_ctx* c, int32_t f, const double* in) {
+    if (!c || !in) return ISPH_E_ARG;
+    int w, comp;
+    bool pair;
+    double* p = field_ptr(c, f, &w, &pair, &comp);
+    if (!p) return fail(c, ISPH_E_ARG, "unknown field");
+    const int m = c->n;
+    if (pair) {
+        std::vector<double> t(2 * (size_t)m);
+        HIPCHK(c, hipMemcpyAsync(t.data(), p, sizeof(double) * 2 * m, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < m; ++i) t[2 * (size_t)i + comp] = in[i];
+        HIPCHK(c, hipMemcpyAsync(p, t.data(), sizeof(double) * 2 * m, hipMemcpyHostToDevice, c->stream));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(p, in, sizeof(double) * w * m, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISPH_OK;
+}
+
+int isph_get_neighbours(isph_ctx* c, int32_t* ptr, int32_t* idx) {
+    if (!c || !ptr) return ISPH_E_ARG;
+    if (!c->have_nbr) return fail(c, ISPH_E_STATE, "no neighbour list");
+    const int m = c->n;
+    std::vector<int> cnt(m);
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), c->cnt, sizeof(int) * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    ptr[0] = 0;
+    int maxc = 0;
+    for (int i = 0; i < m; ++i) {
+        ptr[i + 1] = ptr[i] + cnt[i];
+        maxc = std::max(maxc, cnt[i]);
+    }
+    if (!idx) return ISPH_OK;
+    std::vector<int> ell((size_t)maxc * c->cap);
+    for (int k = 0; k < maxc; ++k)
+        HIPCHK(c, hipMemcpyAsync(ell.data() + (size_t)k * c->cap, c->nbr + (size_t)k * c->cap,
+                                 sizeof(int) * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < m; ++i)
+        for (int k = 0; k < cnt[i]; ++k) idx[ptr[i] + k] = ell[(size_t)k * c->cap + i];
+    return ISPH_OK;
+}
+
+int isph_get_csr(isph_ctx* c, int32_t* rowptr, int32_t* col, double* val, double* rhs) {
+    if (!c || !rowptr) return ISPH_E_ARG;
+    if (!c->assembled) return fail(c, ISPH_E_STATE, "no assembled matrix");
+    const int m = c->n;
+    std::vector<int> cnt(m), dp(m);
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), c->cnt, sizeof(int) * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dp.data(), c->dpos, sizeof(int) * m, hipMemcpyDeviceToHost, c->stream));
+    if (rhs) HIPCHK(c, hipMemcpyAsync(rhs, c->rhs, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    rowptr[0] = 0;
+    int maxc = 0;
+    for (int i = 0; i < m; ++i) {
+        rowptr[i + 1] = rowptr[i] + cnt[i] + 1;
+        maxc = std::max(maxc, cnt[i]);
+    }
+    if (!col && !val) return ISPH_OK;
+    std::vector<int> ell((size_t)maxc * c->cap);
+    std::vector<double> ev((size_t)maxc * c->cap);
+    for (int k = 0; k < maxc; ++k) {
+        HIPCHK(c, hipMemcpyAsync(ell.data() + (size_t)k * c->cap, c->nbr + (size_t)k * c->cap,
+                                 sizeof(int) * m, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(ev.data() + (size_t)k * c->cap, c->val64 + (size_t)k * c->cap,
+                                 sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < m; ++i) {
+        int base = rowptr[i];
+        for (int k = 0; k < cnt[i]; ++k) {
+            int slot = base + (k < dp[i] ? k : k + 1);
+            if (col) col[slot] = ell[(size_t)k * c->cap + i];
+            if (val) val[slot] = ev[(size_t)k * c->cap + i];
+        }
+        if (col) col[base + dp[i]] = i;
+        if (val) val[base + dp[i]] = 1.0;
+    }
+    return ISPH_OK;
+}
+
+int isph_get_scalars(isph_ctx* c, double* umax, double* dt, double* t, int32_t* iters,
+                     int32_t* outer, double* relres) {
+    if (!c) return ISPH_E_ARG;
+    SimScalars s;
+    HIPCHK(c, hipMemcpyAsync(&s, c->ss, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (umax) *umax = s.umax;
+    if (dt) *dt = s.dt;
+    if (t) *t = s.t;
+    if (iters) *iters = c->iters;
+    if (outer) *outer = c->outer;
+    if (relres) *relres = c->relres;
+    return ISPH_OK;
+}
+
+int isph_set_scalars(isph_ctx* c, double umax, double dt, double t) {
+    if (!c) return ISPH_E_ARG;
+    SimScalars s;
+    HIPCHK(c, hipMemcpyAsync(&s, c->ss, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    s.umax = umax;
+    s.dt = dt;
+    s.t = t;
+    HIPCHK(c, hipMemcpyAsync(c->ss, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISPH_OK;
+}
+
+#define STAGE(name, fn)                     \
+    int name(isph_ctx* c) {                 \
+        if (!c) return ISPH_E_ARG;          \
+        int rc = fn(c);                     \
+        if (rc) return rc;                  \
+        return sync_check(c);               \
+    }
+
+static int cellshift_(isph_ctx* c) { return sort_particles(c, false); }
+static int cellshiftalt_(isph_ctx* c) { return sort_particles(c, true); }
+static int resetid_(isph_ctx* c) { (void)c; return ISPH_OK; }
+
+STAGE(isph_projection, run_projection)
+STAGE(isph_cellshift, cellshift_)
+STAGE(isph_resetid, resetid_)
+STAGE(isph_neighbour, run_neighbour)
+STAGE(isph_effpor, run_effpor)
+STAGE(isph_eddyvis, run_eddyvis)
+STAGE(isph_int_vel, run_int_vel)
+STAGE(isph_ppe_assemble, run_assemble)
+STAGE(isph_comp_vel, run_comp_vel)
+STAGE(isph_comp_pos, run_comp_pos)
+STAGE(isph_cellshiftalt, cellshiftalt_)
+STAGE(isph_freesurf, run_freesurf)
+STAGE(isph_opt2_shift, run_shift)
+STAGE(isph_timestep, run_timestep)
+
+int isph_ppe_solve(isph_ctx* c, int32_t* iters, double* relres) {
+    if (!c) return ISPH_E_ARG;
+    int rc = run_solve(c);
+    if (iters) *iters = c->iters;
+    if (relres) *relres = c->relres;
+    prof_collect(c);
+    if (rc) return rc;
+    return sync_check(c);
+}
+
+int isph_ppesolve(isph_ctx* c, int32_t* iters, double* relres) {
+    if (!c) return ISPH_E_ARG;
+    int rc = run_assemble(c);
+    if (rc) return rc;
+    if ((rc = take_errors(c))) return rc;
+    return isph_ppe_solve(c, iters, relres);
+}
+
+int isph_step(isph_ctx* c, int32_t nsteps) {
+    if (!c || nsteps < 0) return ISPH_E_ARG;
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = run_step(c);
+        if (rc) {
+            prof_collect(c);
+            return rc;
+        }
+    }
+    prof_collect(c);
+    return sync_check(c);
+}
+
+int isph_profile_enable(isph_ctx* c, int32_t enable) {
+    if (!c) return ISPH_E_ARG;
+    c->prof.on = enable != 0;
+    for (int k = 0; k < 4; ++k) {
+        c->prof.launches[k] = 0;
+        c->prof.total_ms[k] = 0.0;
+        c->prof.used[k] = 0;
+    }
+    return ISPH_OK;
+}
+
+int isph_get_kernel_time(isph_ctx* c, int32_t kernel, int64_t* launches, double* total_ms) {
+    if (!c || kernel < 0 || kernel > 3) return ISPH_E_ARG;
+    if (launches) *launches = c->prof.launches[kernel];
+    if (total_ms) *total_ms = c->prof.total_ms[kernel];
+    return ISPH_OK;
+}
+
+int isph_copy_bench(isph_ctx* c, int64_t bytes, int32_t reps, double* ms) {
+    if (!c || bytes <= 0 || reps <= 0 || !ms) return ISPH_E_ARG;
+    size_t n2 = (size_t)bytes / sizeof(double2);
+    double2 *a = nullptr, *b = nullptr;
+    HIPCHK(c, hipMalloc((void**)&a, n2 * sizeof(double2)));
+    if (hipMalloc((void**)&b, n2 * sizeof(double2)) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(c, ISPH_E_HIP, "copy_bench allocation failed");
+    }
+    (void)hipMemsetAsync(a, 0, n2 * sizeof(double2), c->stream);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    int grid = 256 * 8 * 4;
+    k_copy<<<grid, BLOCK, 0, c->stream>>>(a, b, n2);
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r) k_copy<<<grid, BLOCK, 0, c->stream>>>(a, b, n2);
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    *ms = (double)t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return ISPH_OK;
+}
+
+}  // extern "C"

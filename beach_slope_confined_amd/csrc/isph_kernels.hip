@@ -1,0 +1,1058 @@
+// isph_kernels.hip -- gfx950 kernels of the ISPH step.
+//
+// One thread per particle / PPE row, 256-thread blocks (4 wave64).  Particle
+// state is structure-of-arrays in HBM, positions and velocities as double2
+// so that a neighbour gather is one 16-byte load.  Neighbour lists and the
+// PPE matrix are column-major ELL with leading dimension `ld` (the particle
+// capacity): entry k of row i sits at k*ld + i, so the 64 lanes of a wave
+// read 64 consecutive entries.  Entries of a row ascend in particle index
+// (the reference's matid order); the PPE diagonal is 1 after Jacobi
+// scaling and is not stored, only its position `dpos` in the row.
+#include "isph_kernels.h"
+
+namespace isph {
+
+// ---------------------------------------------------------------------------
+// reductions in a fixed tree: wave64 butterfly (xor 1,2,..,32), the four
+// waves as (w0+w1)+(w2+w3), block partials finished by the last block
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) v = v + __shfl_xor(v, s, 64);
+    return v;
+}
+
+// Logical block of a solver launch: physical blocks are dealt round-robin
+// to the 8 XCDs, so XCD x gets the contiguous logical range
+// [x*per, (x+1)*per) and its L2 holds the vector entries its rows gather.
+__device__ __forceinline__ int xcd_block(int bid, int nblocks) {
+    int per = nblocks >> 3;  // nblocks is a multiple of 8
+    return (bid & 7) * per + (bid >> 3);
+}
+
+template <int NQ>
+__device__ __forceinline__ bool reduce_tree(const double (&v)[NQ], double* part, int nbp, int lb,
+                                            uint32_t* counter, double (&out)[NQ]) {
+    __shared__ double red[NQ][4];
+    __shared__ int amlast;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double s = wave_sum(v[q]);
+        if (lane == 0) red[q][w] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            __hip_atomic_store(&part[q * nbp + lb], (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        uint32_t t = atomicAdd(counter, 1u);
+        amlast = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!amlast) return false;
+    __threadfence();
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double s = 0.0;
+        for (int k = threadIdx.x; k < nbp; k += BLOCK)
+            s = s + __hip_atomic_load(&part[q * nbp + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s = wave_sum(s);
+        __syncthreads();
+        if (lane == 0) red[q][w] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) out[q] = (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]);
+    if (threadIdx.x == 0) atomicExch(counter, 0u);
+    return true;
+}
+
+__device__ __forceinline__ void raise_err(uint32_t* err, uint32_t bit) { atomicOr(err, bit); }
+
+// ---------------------------------------------------------------------------
+// cell binning: stable counting sort by key = 2*cell + fluid bit
+// (cellshift/cellshiftalt re-binning, integrator.f90:32-391, + resetid,
+// isph.f90:88-113)
+// ---------------------------------------------------------------------------
+__device__ inline int cell_of(const GridInfo& g, double x, double y, bool& oob) {
+    double fx = floor((x - g.xmin) / g.cs);
+    double fy = floor((y - g.ymin) / g.cs);
+    int cx, cy;
+    if (!(fx >= 0.0)) { cx = 1; oob = true; }
+    else if (fx > (double)(g.nx - 3)) { cx = g.nx - 2; oob = true; }
+    else cx = (int)fx + 1;
+    if (!(fy >= 0.0)) { cy = 1; oob = true; }
+    else if (fy > (double)(g.ny - 3)) { cy = g.ny - 2; oob = true; }
+    else cy = (int)fy + 1;
+    return cx * g.ny + cy;
+}
+
+__global__ void k_bin_keys(int n, GridInfo g, const double2* __restrict__ pos,
+                           const int* __restrict__ tid, const int* __restrict__ flags, int sink,
+                           double xlcut, double ytcut, int* __restrict__ key,
+                           int* __restrict__ bincnt, int* __restrict__ rank, uint32_t* err) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double2 p = pos[i];
+    int fl = tid[i] == 3;
+    int k;
+    if (sink && fl && !(flags[i] & ISPH_FLAG_BUFFER) && p.x <= xlcut && p.y > ytcut) {
+        k = (int)g.nbins;
+    } else {
+        bool oob = false;
+        k = 2 * cell_of(g, p.x, p.y, oob) + fl;
+        if (oob) raise_err(err, ERR_DOMAIN);
+    }
+    key[i] = k;
+    rank[i] = atomicAdd(&bincnt[k], 1);
+}
+
+// exclusive scan of int32 in place: per-block scan, scan of block sums, add
+__global__ void k_scan_blocks(int n, int* __restrict__ data, int* __restrict__ bsum) {
+    __shared__ int s[SCAN_BLOCK];
+    int i = blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    int v = i < n ? data[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+        int t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n) data[i] = s[threadIdx.x] - v;
+    if (threadIdx.x == SCAN_BLOCK - 1) bsum[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ void k_scan_sums(int nb, int* __restrict__ bsum) {
+    __shared__ int s[SCAN_BLOCK];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += SCAN_BLOCK) {
+        int i = base + threadIdx.x;
+        int v = i < nb ? bsum[i] : 0;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+            int t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+            __syncthreads();
+            s[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = carry + s[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == SCAN_BLOCK - 1) carry += s[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+__global__ void k_scan_add(int n, int* __restrict__ data, const int* __restrict__ bsum) {
+    int i = blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    if (i < n) data[i] += bsum[blockIdx.x];
+}
+
+__global__ void k_bin_scatter(int n, const int* __restrict__ key, const int* __restrict__ rank,
+                              const int* __restrict__ bstart, int* __restrict__ perm) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    perm[bstart[key[i]] + rank[i]] = i;
+}
+
+// restore stability: within each bin, ascending previous position
+__global__ void k_bin_fixup(int nbins, const int* __restrict__ bstart, int* __restrict__ perm) {
+    int b = blockIdx.x * BLOCK + threadIdx.x;
+    if (b >= nbins) return;
+    int lo = bstart[b], hi = bstart[b + 1];
+    for (int a = lo + 1; a < hi; ++a) {
+        int v = perm[a];
+        int c = a - 1;
+        while (c >= lo && perm[c] > v) {
+            perm[c + 1] = perm[c];
+            --c;
+        }
+        perm[c + 1] = v;
+    }
+}
+
+__global__ void k_gather_state(int n, const int* __restrict__ perm, StateView src, StateView dst,
+                               const int* __restrict__ key, int* __restrict__ key_sorted) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    int o = perm[i];
+    dst.pos[i] = src.pos[o];
+    dst.vel[i] = src.vel[o];
+    dst.sp[i] = src.sp[o];
+    dst.sv[i] = src.sv[o];
+    dst.mass[i] = src.mass[o];
+    dst.rho[i] = src.rho[o];
+    dst.oden[i] = src.oden[o];
+    dst.p[i] = src.p[o];
+    dst.por[i] = src.por[o];
+    dst.R[i] = src.R[o];
+    dst.nut[i] = src.nut[o];
+    dst.tke[i] = src.tke[o];
+    dst.varts[i] = src.varts[o];
+    dst.velm[i] = src.velm[o];
+    dst.tid[i] = src.tid[o];
+    dst.flags[i] = src.flags[o];
+    dst.pid[i] = src.pid[o];
+    key_sorted[i] = key[o];
+}
+
+// ---------------------------------------------------------------------------
+// projection, isph.f90:16-47
+// ---------------------------------------------------------------------------
+__global__ void k_projection(int n, StateView s, const SimScalars* ss) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (s.tid[i] == 3 && !(s.flags[i] & ISPH_FLAG_DEAD)) {
+        const double dt = ss->dt;
+        double2 p = s.pos[i], v = s.vel[i];
+        double n_ = s.por[i];
+        s.sp[i] = p;
+        p.x = p.x + (dt * v.x) / n_;
+        p.y = p.y + (dt * v.y) / n_;
+        s.pos[i] = p;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// neighbour + nsearch2, search.f90:13-399: list, KGC matrix A (:77-85),
+// divr (:272), L = inv(A) (:276-278), free flag (:302-304)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLOCK) k_neighbour(int n, GridInfo g, KernelConsts kc,
+                                                     double thr, int maxnbr, size_t ld,
+                                                     StateView s, const int* __restrict__ key,
+                                                     const int* __restrict__ bstart,
+                                                     int* __restrict__ nbr, int* __restrict__ cnt,
+                                                     double* __restrict__ L,
+                                                     double* __restrict__ divr, uint32_t* err) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int cell = key[i] >> 1;
+    const int cx = cell / g.ny, cy = cell - cx * g.ny;
+    const double2 pi = s.pos[i];
+    int c = 0;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int dc = -1; dc <= 1; ++dc) {
+        const int col = (cx + dc) * g.ny + cy;
+        const int lo = bstart[2 * (col - 1)], hi = bstart[2 * (col + 1) + 2];
+        for (int j = lo; j < hi; ++j) {
+            if (j == i) continue;
+            const double2 pj = s.pos[j];
+            double dx = pi.x - pj.x, dy = pi.y - pj.y;
+            double r = sqrt(dx * dx + dy * dy);
+            if (!(r < kc.twoh)) continue;
+            if (c < maxnbr) nbr[(size_t)c * ld + i] = j;
+            c++;
+            double wx, wy;
+            kern_gradw(kc, r, dx, dy, wx, wy);
+            const double mj = s.mass[j], rj = s.rho[j];
+            double res1 = (mj * (pj.x - pi.x)) / rj;
+            double res3 = (mj * (pj.y - pi.y)) / rj;
+            a0 = a0 + wx * res1;
+            a1 = a1 + wy * res1;
+            a2 = a2 + wx * res3;
+            a3 = a3 + wy * res3;
+        }
+    }
+    if (c > maxnbr) {
+        raise_err(err, ERR_CAPACITY);
+        c = maxnbr;
+    }
+    cnt[i] = c;
+    int fl = s.flags[i] & ~ISPH_FLAG_FREE;
+    double dv = 0.0;
+    double l0 = 1.0, l1 = 0.0, l2 = 0.0, l3 = 1.0;
+    if (s.tid[i] == 3) {
+        fl &= ~ISPH_FLAG_VICINITY;
+        dv = a0 + a3;
+        l0 = a0; l1 = a1; l2 = a2; l3 = a3;
+        invert2(l0, l1, l2, l3);
+        if (dv <= thr && !(s.flags[i] & ISPH_FLAG_BUFFER)) fl |= ISPH_FLAG_FREE;
+    }
+    divr[i] = dv;
+    double2* L2 = reinterpret_cast<double2*>(L);
+    L2[2 * (size_t)i] = make_double2(l0, l1);
+    L2[2 * (size_t)i + 1] = make_double2(l2, l3);
+    s.flags[i] = fl;
+}
+
+// ---------------------------------------------------------------------------
+// effpor, porous.f90:107-224 (+ porsearch :32-105)
+// ---------------------------------------------------------------------------
+__global__ void k_effpor(int n, GridInfo g, KernelConsts kc, PorousParams pp, StateView s,
+                         const double2* __restrict__ spos, const double* __restrict__ sm,
+                         const double* __restrict__ sr, const int* __restrict__ sstart,
+                         uint32_t* err) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    s.R[i] = 0.0;
+    if (s.tid[i] != 3) return;
+    const double2 pi = s.pos[i];
+    bool oob = false;
+    const int cell = cell_of(g, pi.x, pi.y, oob);
+    if (oob) raise_err(err, ERR_DOMAIN);
+    const int cx = cell / g.ny, cy = cell - cx * g.ny;
+    double sf = 0.0;
+    const double cpor = 1.0 - pp.por * 1.0;
+    for (int dc = -1; dc <= 1; ++dc) {
+        const int col = (cx + dc) * g.ny + cy;
+        const int lo = sstart[col - 1], hi = sstart[col + 2];
+        for (int q = lo; q < hi; ++q) {
+            const double2 ps = spos[q];
+            double dx = pi.x - ps.x, dy = pi.y - ps.y;
+            double r = sqrt(dx * dx + dy * dy);
+            if (r < kc.twoh) sf = sf + ((cpor * kern_w(kc, r)) * sm[q]) / sr[q];
+        }
+    }
+    if (sf > (1.0 - pp.por)) sf = 1.0 - pp.por;
+    const double nn = 1.0 - sf;
+    s.por[i] = nn;
+    const double oden = s.oden[i];
+    s.rho[i] = nn * oden;
+    const double n3 = (nn * nn) * nn;
+    const double2 v = s.vel[i];
+    const double vm = sqrt(v.x * v.x + v.y * v.y);
+    const double omn = 1.0 - nn;
+    double R;
+    if (pp.drag_law == ISPH_DRAG_REFERENCE) {
+        double fch = 1.750 / sqrt(150.0 * n3);
+        double t1 = (((visc(pp.visc_law, pp.nu, oden) / oden) * pp.alpha) * (omn * omn)) /
+                    (n3 * (pp.d50 * pp.d50));
+        double t2 = (((fch * vm) * sqrt(pp.alpha)) * omn) / (sqrt(n3) * pp.d50);
+        R = (-t1) - t2;
+    } else {
+        double t1 = ((pp.alpha * pp.nu) * (omn * omn)) / (n3 * (pp.d50 * pp.d50));
+        double t2 = ((pp.beta * vm) * omn) / (n3 * pp.d50);
+        R = (-t1) - t2;
+    }
+    s.R[i] = R;
+}
+
+__device__ __forceinline__ void load_L(const double* L, int i, double& l0, double& l1, double& l2,
+                                       double& l3) {
+    const double2* L2 = reinterpret_cast<const double2*>(L);
+    double2 a = L2[2 * (size_t)i], b = L2[2 * (size_t)i + 1];
+    l0 = a.x; l1 = a.y; l2 = b.x; l3 = b.y;
+}
+
+// ---------------------------------------------------------------------------
+// eddyvis, turbulence.f90:12-110
+// ---------------------------------------------------------------------------
+__global__ void k_eddyvis(int n, KernelConsts kc, double dl, size_t ld, StateView s,
+                          const int* __restrict__ nbr, const int* __restrict__ cnt,
+                          const double* __restrict__ L) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n || s.tid[i] != 3) return;
+    double l0, l1, l2, l3;
+    load_L(L, i, l0, l1, l2, l3);
+    const double2 pi = s.pos[i], vi = s.vel[i];
+    const double ni = s.por[i];
+    const double ui = vi.x / ni, wi = vi.y / ni;
+    double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    const int c = cnt[i];
+    for (int k = 0; k < c; ++k) {
+        const int j = nbr[(size_t)k * ld + i];
+        double r, wx, wy;
+        pair_geom(kc, pi, s.pos[j], r, wx, wy);
+        double gx = l0 * wx + l1 * wy;
+        double gy = l2 * wx + l3 * wy;
+        const double2 vj = s.vel[j];
+        const double nj = s.por[j], mj = s.mass[j], rj = s.rho[j];
+        double du = vj.x / nj - ui;
+        double dv = vj.y / nj - wi;
+        t1 = t1 + ((mj * du) * gx) / rj;
+        t2 = t2 + ((mj * dv) * gy) / rj;
+        t3 = t3 + ((0.50 * mj) * (du * gy + dv * gx)) / rj;
+    }
+    const double cs = 0.150, cv = 0.080, ce = 1.0;
+    double t4 = sqrt(2.0 * (t1 * t1 + t2 * t2) + (2.0 * t3) * (2.0 * t3));
+    s.nut[i] = ((cs * dl) * (cs * dl)) * t4;
+    s.tke[i] = (cv / ce) * ((dl * t4) * (dl * t4));
+}
+
+// ---------------------------------------------------------------------------
+// int_vel, integrator.f90:458-713: (i) varts of free particles (:467-506)
+// ---------------------------------------------------------------------------
+__global__ void k_varts(int n, KernelConsts kc, double dl, size_t ld, StateView s,
+                        const int* __restrict__ nbr, const int* __restrict__ cnt,
+                        const double* __restrict__ L, const SimScalars* ss) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n || !(s.flags[i] & ISPH_FLAG_FREE)) return;
+    double l0, l1, l2, l3;
+    load_L(L, i, l0, l1, l2, l3);
+    const double2 pi = s.pos[i], vi = s.vel[i];
+    const double ni = s.por[i];
+    const double ui = vi.x / ni, wi = vi.y / ni;
+    double t1 = 0.0, t2 = 0.0;
+    const int c = cnt[i];
+    for (int k = 0; k < c; ++k) {
+        const int j = nbr[(size_t)k * ld + i];
+        double r, wx, wy;
+        pair_geom(kc, pi, s.pos[j], r, wx, wy);
+        double gx = l0 * wx + l1 * wy;
+        double gy = l2 * wx + l3 * wy;
+        const double2 vj = s.vel[j];
+        const double nj = s.por[j];
+        const double vol = s.mass[j] / s.rho[j];
+        t1 = t1 + ((vj.x / nj - ui) * gx) * vol;
+        t2 = t2 + ((vj.y / nj - wi) * gy) * vol;
+    }
+    const double ps = 1.0;
+    s.varts[i] = (((ps * (dl * dl)) * sqrt(t1 * t1 + t2 * t2)) * sqrt(vi.x * vi.x + vi.y * vi.y)) /
+                 ((ss->umax + TINY6) * ni);
+}
+
+// (ii) v* of fluid particles (:509-698)
+__global__ void __launch_bounds__(BLOCK) k_vstar(int n, KernelConsts kc, PorousParams pp,
+                                                 double lam, double g, size_t ld, StateView s,
+                                                 const int* __restrict__ nbr,
+                                                 const int* __restrict__ cnt,
+                                                 const double* __restrict__ L,
+                                                 const SimScalars* ss) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int fi = s.flags[i];
+    const double dt = ss->dt;
+    if (s.tid[i] == 3 && !(fi & ISPH_FLAG_BUFFER) && !(fi & ISPH_FLAG_DEAD)) {
+        double l0, l1, l2, l3;
+        load_L(L, i, l0, l1, l2, l3);
+        const double2 pi = s.pos[i], vi = s.vel[i];
+        const double ni = s.por[i], oi = s.oden[i];
+        const double mui = visc(pp.visc_law, pp.nu, oi);
+        const double nuti = s.nut[i], tkei = s.tke[i];
+        const double w0 = kern_w0(kc);
+        double sx = 0.0, sy = 0.0;
+        const int c = cnt[i];
+        for (int k = 0; k < c; ++k) {
+            const int j = nbr[(size_t)k * ld + i];
+            const double2 pj = s.pos[j];
+            double r, wx, wy;
+            pair_geom(kc, pi, pj, r, wx, wy);
+            double gx = l0 * wx + l1 * wy;
+            double gy = l2 * wx + l3 * wy;
+            const int tj = s.tid[j];
+            double cmu;
+            if (tj == 3) {
+                const double oj = s.oden[j];
+                const double muj = visc(pp.visc_law, pp.nu, oj);
+                cmu = (((2.0 * mui) * muj) * ((1.0 / oi) * (1.0 / oj))) / ((mui / oi) + (muj / oj));
+            } else {
+                cmu = mui * (1.0 / oi);
+            }
+            double vart;
+            if (fi & ISPH_FLAG_FREE) vart = s.varts[i];
+            else if ((s.flags[j] & ISPH_FLAG_FREE) && !(fi & ISPH_FLAG_BUFFER))
+                vart = (kern_w(kc, r) * s.varts[j]) / w0;
+            else vart = 0.0;
+            const double pvol = s.mass[j] / s.rho[j];
+            const double pdist = 1.0 / (r * r + lam);
+            const double dxp = pi.x - pj.x, dyp = pi.y - pj.y;
+            const double2 vj = s.vel[j];
+            double dvx = vi.x - vj.x;
+            double t1 = ((((2.0 * pvol) * dvx) * dxp) * gx) * pdist;
+            double t2 = ((((2.0 * pvol) * dvx) * dyp) * gy) * pdist;
+            sx = sx + (t1 + t2) * (cmu + vart);
+            double nutj = 0.0;
+            if (tj > 2) {
+                nutj = s.nut[j];
+                if (nuti < TINY6 && nutj < TINY6) sx = sx + ((t1 + t2) * (nuti + nutj)) * 0.50;
+                else sx = sx + (t1 + t2) * (((2.0 * nuti) * nutj) / (nuti + nutj));
+            }
+            double dvy = vi.y - vj.y;
+            t1 = ((((2.0 * pvol) * dvy) * dxp) * gx) * pdist;
+            t2 = ((((2.0 * pvol) * dvy) * dyp) * gy) * pdist;
+            sy = sy + (t1 + t2) * (cmu + vart);
+            if (tj > 2) {
+                if (nuti < TINY6 && nutj < TINY6) sy = sy + ((t1 + t2) * (nuti + nutj)) * 0.50;
+                else sy = sy + (t1 + t2) * (((2.0 * nuti) * nutj) / (nuti + nutj));
+            }
+            const double dtk = s.tke[j] - tkei;
+            t1 = (pvol * dtk) * gx;
+            sx = sx - ((t1 * ni) * 2.0) / 3.0;
+            t1 = (pvol * dtk) * gy;
+            sy = sy - ((t1 * ni) * 2.0) / 3.0;
+        }
+        const double den = 1.0 - (dt * s.R[i]) * ni;
+        double vxs = (vi.x + dt * sx) / den;
+        double vys = (vi.y + dt * sy) / den;
+        vys = vys + (g * ni) * dt;
+        s.sv[i] = make_double2(vxs, vys);
+    }
+    if (fi & ISPH_FLAG_BUFFER) s.sv[i] = make_double2(s.vel[i].x, (g * s.por[i]) * dt);
+}
+
+// ---------------------------------------------------------------------------
+// PPE assembly, isph.f90:123-456 (row rules :145-421, diagonal position
+// :167-176,425-426, Jacobi scaling :434-442)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLOCK) k_assemble(int n, KernelConsts kc, double lam,
+                                                    double lampd, size_t ld, StateView s,
+                                                    const int* __restrict__ nbr,
+                                                    const int* __restrict__ cnt,
+                                                    const double* __restrict__ L,
+                                                    double* __restrict__ val,
+                                                    float* __restrict__ val32,
+                                                    int* __restrict__ dpos, double* __restrict__ rhs,
+                                                    const SimScalars* ss, uint32_t* err) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double dt = ss->dt;
+    const int ti = s.tid[i];
+    const bool freei = (s.flags[i] & ISPH_FLAG_FREE) != 0;
+    double li0, li1, li2, li3;
+    load_L(L, i, li0, li1, li2, li3);
+    const double2 pi = s.pos[i], svi = s.sv[i];
+    const double ni = s.por[i], rhoi = s.rho[i], mi = s.mass[i];
+    const double rho_i = rhoi * (1.0 / ni);
+    const int c = cnt[i];
+    int dp = 0;
+    double dia = 0.0, b = 0.0;
+    for (int k = 0; k < c; ++k) {
+        const int j = nbr[(size_t)k * ld + i];
+        if (j < i) dp = k + 1;
+        double v = 0.0;
+        const int tj = s.tid[j];
+        const double lamp = (freei && tj == 3) ? (1.0 + s.oden[j] / lampd) : 1.0;
+        const double nj = s.por[j], rhoj = s.rho[j];
+        const double rho_j = rhoj * (1.0 / nj);
+        const double pvol = s.mass[j] * (1.0 / rhoj);
+        const double2 pj = s.pos[j];
+        double ddx = pj.x - pi.x, ddy = pj.y - pi.y;
+        double rr = sqrt(ddx * ddx + ddy * ddy);
+        double pdist = 1.0 / (rr * rr + lam);
+        double r, wx, wy;
+        pair_geom(kc, pi, pj, r, wx, wy);
+        const double dxp = pi.x - pj.x, dyp = pi.y - pj.y;
+        double t1, t2;
+        if (ti == 3) {
+            const double rho_ij = (((2.0 * rhoj) * rhoi) * (1.0 / (nj * ni))) *
+                                  (1.0 / (rhoj * (1.0 / nj) + rhoi * (1.0 / ni)));
+            double gx = li0 * wx + li1 * wy;
+            double gy = li2 * wx + li3 * wy;
+            t1 = (((2.0 * pvol) * gx) * dxp) * pdist;
+            t2 = (((2.0 * pvol) * gy) * dyp) * pdist;
+            v = -(t1 + t2);
+            dia = dia + (t1 + t2) * lamp;
+            const double2 svj = s.sv[j];
+            if (tj == 3) {
+                t1 = (pvol * ((svj.x * rho_ij) / nj - (svi.x * rho_ij) / ni)) * gx;
+                t2 = (pvol * ((svj.y * rho_ij) / nj - (svi.y * rho_ij) / ni)) * gy;
+            } else {
+                t1 = (pvol * ((svj.x * rho_i) / ni - (svi.x * rho_i) / ni)) * gx;
+                t2 = (pvol * ((svj.y * rho_i) / ni - (svi.y * rho_i) / ni)) * gy;
+            }
+            b = b + (t1 + t2) / dt;
+        } else if (ti == 2) {
+            t1 = (((2.0 * pvol) * wx) * dxp) * pdist;
+            t2 = (((2.0 * pvol) * wy) * dyp) * pdist;
+            if (tj <= 2) {
+                v = -(t1 + t2);
+                dia = dia + (t1 + t2);
+            }
+        } else if (ti == 1) {
+            if (tj == 3) {
+                double lj0, lj1, lj2, lj3;
+                load_L(L, j, lj0, lj1, lj2, lj3);
+                double gjx = lj0 * wx + lj1 * wy;
+                double gjy = lj2 * wx + lj3 * wy;
+                t1 = ((((2.0 * mi) * gjx) * dxp) * pdist) / rhoi;
+                t2 = ((((2.0 * mi) * gjy) * dyp) * pdist) / rhoi;
+                v = (-(t1 + t2)) * 1.0;
+                dia = dia + (t1 + t2);
+                const double2 svj = s.sv[j];
+                t1 = ((mi * ((svj.x * rho_j) / nj - (svi.x * rho_j) / ni)) * gjx) / rhoi;
+                t2 = ((mi * ((svj.y * rho_j) / nj - (svi.y * rho_j) / ni)) * gjy) / rhoi;
+                b = b + (t1 + t2) / dt;
+            } else if (tj <= 2) {
+                t1 = (((2.0 * pvol) * wx) * dxp) * pdist;
+                t2 = (((2.0 * pvol) * wy) * dyp) * pdist;
+                v = -(t1 + t2);
+                dia = dia + (t1 + t2);
+            }
+        }
+        val[(size_t)k * ld + i] = v;
+    }
+    if (!(dia != 0.0) || !isfinite(dia)) raise_err(err, ERR_SINGULAR);
+    for (int k = 0; k < c; ++k) {
+        double v = val[(size_t)k * ld + i] / dia;
+        val[(size_t)k * ld + i] = v;
+        if (val32) val32[(size_t)k * ld + i] = (float)v;
+    }
+    dpos[i] = dp;
+    rhs[i] = b / dia;
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB, recurrences of solver.f90:162-213 (van der Vorst 1992), on the
+// Jacobi-scaled matrix; T = double (FP64 solve) or float (mixed-precision
+// inner solve, dot products accumulated in double).
+// ---------------------------------------------------------------------------
+template <class T, class F>
+__device__ __forceinline__ T spmv_row(int i, int c, int dp, const int* __restrict__ nbr,
+                                      const T* __restrict__ val, size_t ld, T xi, F&& xj) {
+    T s = (T)0;
+    for (int k = 0; k < c; ++k) {
+        if (k == dp) s = s + xi;
+        const int j = nbr[(size_t)k * ld + i];
+        s = s + val[(size_t)k * ld + i] * xj(j);
+    }
+    if (dp == c) s = s + xi;
+    return s;
+}
+
+// init: res = A x0, rs = b - res, rhat = rs; dots (b,b), (rs,rs)
+__global__ void __launch_bounds__(BLOCK) k_bicg_init(SolveArgs a, SolverScalars* sc) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    double vb = 0.0, vr = 0.0;
+    if (i < a.n) {
+        const double* x = a.x;
+        double s = spmv_row<double>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, x[i],
+                                    [&](int j) { return x[j]; });
+        double bi = a.b[i];
+        double r = bi - s;
+        a.rs[i] = r;
+        a.rhat[i] = r;
+        vb = bi * bi;
+        vr = r * r;
+    }
+    double v[2] = {vb, vr}, out[2];
+    if (reduce_tree<2>(v, a.part, gridDim.x, lb, &a.counters[0], out) && threadIdx.x == 0) {
+        sc->bb = out[0];
+        sc->nb = sqrt(out[0]);
+        sc->rr = out[1];
+        sc->prevr = out[1];
+        sc->iters = 0;
+        sc->first = 1;
+        sc->status = 0;
+        if (out[0] == 0.0) {
+            sc->done = DONE_ZERO_RHS;
+            sc->relres = 0.0;
+        } else if (sqrt(out[1]) <= a.tol * sc->nb) {
+            sc->done = DONE_CONVERGED;
+            sc->relres = sqrt(out[1]) / sc->nb;
+        } else {
+            sc->done = 0;
+        }
+    }
+}
+
+// iteration step 1: p = rs + beta (p - w mm1) (or p = rs after a (re)start),
+// mm1 = A p, alpha = (rhat, rs) / (mm1, rhat)
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_bicg_1(SolveArgsT<T> a, SolverScalars* sc,
+                                                  const T* __restrict__ p_old,
+                                                  const T* __restrict__ mm1_old,
+                                                  T* __restrict__ p_new, T* __restrict__ mm1_new) {
+    if (sc->done) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    const bool first = sc->first != 0;
+    const T beta = (T)sc->betas, w = (T)sc->wi;
+    const T* rs = a.rs;
+    auto pn = [&](int j) -> T {
+        return first ? rs[j] : rs[j] + beta * (p_old[j] - w * mm1_old[j]);
+    };
+    double v = 0.0;
+    if (i < a.n) {
+        T pi = pn(i);
+        T s = spmv_row<T>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, pi, pn);
+        p_new[i] = pi;
+        mm1_new[i] = s;
+        v = (double)s * (double)a.rhat[i];
+    }
+    double vv[1] = {v}, out[1];
+    if (reduce_tree<1>(vv, a.part, gridDim.x, lb, &a.counters[1], out) && threadIdx.x == 0) {
+        sc->alpha = sc->prevr / out[0];
+    }
+}
+
+// iteration step 2: si = rs - alpha mm1, mm2 = A si, w = (mm2,si)/(mm2,mm2)
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_bicg_2(SolveArgsT<T> a, SolverScalars* sc,
+                                                  const T* __restrict__ mm1) {
+    if (sc->done) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    const T alpha = (T)sc->alpha;
+    const T* rs = a.rs;
+    auto sfn = [&](int j) -> T { return rs[j] - alpha * mm1[j]; };
+    double v0 = 0.0, v1 = 0.0;
+    if (i < a.n) {
+        T si = sfn(i);
+        T s = spmv_row<T>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, si, sfn);
+        a.si[i] = si;
+        a.mm2[i] = s;
+        v0 = (double)s * (double)si;
+        v1 = (double)s * (double)s;
+    }
+    double vv[2] = {v0, v1}, out[2];
+    if (reduce_tree<2>(vv, a.part, gridDim.x, lb, &a.counters[2], out) && threadIdx.x == 0) {
+        sc->wi = out[0] / out[1];
+    }
+}
+
+// iteration step 3: x += alpha p + w si, rs = si - w mm2,
+// beta = ((rs,rhat)/prevr)(alpha/w), stop tests
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_bicg_3(SolveArgsT<T> a, SolverScalars* sc,
+                                                  const T* __restrict__ p, int inner) {
+    if (sc->done) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    const T alpha = (T)sc->alpha, w = (T)sc->wi;
+    double v0 = 0.0, v1 = 0.0;
+    if (i < a.n) {
+        T si = a.si[i];
+        T x = (a.xs[i] + alpha * p[i]) + w * si;
+        T r = si - w * a.mm2[i];
+        a.xs[i] = x;
+        a.rs[i] = r;
+        v0 = (double)r * (double)a.rhat[i];
+        v1 = (double)r * (double)r;
+    }
+    double vv[2] = {v0, v1}, out[2];
+    if (reduce_tree<2>(vv, a.part, gridDim.x, lb, &a.counters[3], out) && threadIdx.x == 0) {
+        double betas = (out[0] / sc->prevr) * (sc->alpha / sc->wi);
+        sc->betas = betas;
+        sc->prevr = out[0];
+        sc->rr = out[1];
+        sc->iters = sc->iters + 1;
+        sc->total = sc->total + 1;
+        sc->first = 0;
+        if (!isfinite(betas) || !isfinite(out[1])) {
+            sc->done = inner ? DONE_INNER : DONE_FAILED;
+            if (!inner) sc->status = ERR_NOCONV;
+        } else if (!inner && sqrt(out[1]) <= a.tol * sc->nb) {
+            sc->done = DONE_RECURRENCE;
+        } else if (inner && sqrt(out[1]) <= a.tol * sqrt(sc->rr0)) {
+            sc->done = DONE_INNER;
+        } else if (sc->total >= a.maxit || (inner && sc->iters >= a.inner_maxit)) {
+            sc->done = inner ? DONE_INNER : DONE_FAILED;
+            if (!inner) sc->status = ERR_NOCONV;
+        }
+    }
+}
+
+// true residual r = b - A x after the recurrence residual passed; restart
+// from it when it did not
+__global__ void __launch_bounds__(BLOCK) k_bicg_true(SolveArgs a, SolverScalars* sc) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (i < a.n) {
+        const double* x = a.x;
+        double s = spmv_row<double>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, x[i],
+                                    [&](int j) { return x[j]; });
+        double r = a.b[i] - s;
+        a.rs[i] = r;
+        a.rhat[i] = r;
+        v = r * r;
+    }
+    double vv[1] = {v}, out[1];
+    if (reduce_tree<1>(vv, a.part, gridDim.x, lb, &a.counters[4], out) && threadIdx.x == 0) {
+        double tr = out[0];
+        if (sqrt(tr) <= a.tol * sc->nb) {
+            sc->done = DONE_CONVERGED;
+            sc->relres = sqrt(tr) / sc->nb;
+        } else {
+            sc->rr = tr;
+            sc->prevr = tr;
+            sc->first = 1;
+            sc->done = 0;
+            if (sc->total >= a.maxit) {
+                sc->done = DONE_FAILED;
+                sc->status = ERR_NOCONV;
+                sc->relres = sqrt(tr) / sc->nb;
+            }
+        }
+    }
+}
+
+// mixed precision, outer step: r64 = b - A64 x64, ||r64||; converged?
+__global__ void __launch_bounds__(BLOCK) k_mixed_resid(SolveArgs a, SolverScalars* sc,
+                                                       double* __restrict__ r64) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    double vb = 0.0, vr = 0.0;
+    if (i < a.n) {
+        const double* x = a.x;
+        double s = spmv_row<double>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, x[i],
+                                    [&](int j) { return x[j]; });
+        double bi = a.b[i];
+        double r = bi - s;
+        r64[i] = r;
+        vb = bi * bi;
+        vr = r * r;
+    }
+    double vv[2] = {vb, vr}, out[2];
+    if (reduce_tree<2>(vv, a.part, gridDim.x, lb, &a.counters[5], out) && threadIdx.x == 0) {
+        sc->bb = out[0];
+        sc->nb = sqrt(out[0]);
+        sc->nr = sqrt(out[1]);
+        sc->relres = sc->bb == 0.0 ? 0.0 : sc->nr / sc->nb;
+        if (out[0] == 0.0) sc->done = DONE_ZERO_RHS;
+        else if (sc->nr <= a.tol * sc->nb) sc->done = DONE_CONVERGED;
+        else sc->done = 0;
+    }
+}
+
+// mixed precision, inner start: rs = float(r64/||r64||), rhat = rs, x32 = 0
+__global__ void __launch_bounds__(BLOCK) k_mixed_start(SolveArgsT<float> f, SolverScalars* sc,
+                                                       const double* __restrict__ r64) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (i < f.n) {
+        const double inv = 1.0 / sc->nr;
+        float r = (float)(r64[i] * inv);
+        f.rs[i] = r;
+        f.rhat[i] = r;
+        f.xs[i] = 0.0f;
+        v = (double)r * (double)r;
+    }
+    double vv[1] = {v}, out[1];
+    if (reduce_tree<1>(vv, f.part, gridDim.x, lb, &f.counters[6], out) && threadIdx.x == 0) {
+        sc->prevr = out[0];
+        sc->rr0 = out[0];
+        sc->first = 1;
+        sc->iters = 0;
+        sc->done = 0;
+    }
+}
+
+// mixed precision, refinement: x64 += double(x32) * ||r64||
+__global__ void k_mixed_update(int n, double* __restrict__ x, const float* __restrict__ d,
+                               const SolverScalars* sc) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    x[i] = x[i] + (double)d[i] * sc->nr;
+}
+
+// ---------------------------------------------------------------------------
+// comp_vel, integrator.f90:715-865
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLOCK) k_comp_vel(int n, KernelConsts kc, size_t ld,
+                                                    StateView s, const int* __restrict__ nbr,
+                                                    const int* __restrict__ cnt,
+                                                    const double* __restrict__ L,
+                                                    const SimScalars* ss, uint32_t* err) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int fi = s.flags[i];
+    const double dt = ss->dt;
+    if (s.tid[i] == 3 && !(fi & ISPH_FLAG_BUFFER) && !(fi & ISPH_FLAG_DEAD)) {
+        double l0, l1, l2, l3;
+        load_L(L, i, l0, l1, l2, l3);
+        const double2 pi = s.pos[i];
+        const double ni = s.por[i], rhoi = s.rho[i], p_i = s.p[i];
+        double vx = 0.0, vy = 0.0;
+        const int c = cnt[i];
+        for (int k = 0; k < c; ++k) {
+            const int j = nbr[(size_t)k * ld + i];
+            double r, wx, wy;
+            pair_geom(kc, pi, s.pos[j], r, wx, wy);
+            double gx = l0 * wx + l1 * wy;
+            double gy = l2 * wx + l3 * wy;
+            const double vj = s.mass[j] / s.rho[j];
+            double t1;
+            if (s.tid[j] == 3) {
+                const double nj = s.por[j], rhoj = s.rho[j], pj = s.p[j];
+                double q = ((pj * rhoi) / ni) + ((p_i * rhoj) / nj);
+                double den = rhoi * ((rhoi / ni) + (rhoj / nj));
+                t1 = (2.0 * (((vj * (ni * ni)) * q) * gx)) / den;
+                vx = vx - t1 * dt;
+                t1 = (2.0 * (((vj * (ni * ni)) * q) * gy)) / den;
+                vy = vy - t1 * dt;
+            } else {
+                double q = s.p[j] + p_i;
+                t1 = (((vj * (ni * ni)) * q) * gx) / rhoi;
+                vx = vx - t1 * dt;
+                t1 = (((vj * (ni * ni)) * q) * gy) / rhoi;
+                vy = vy - t1 * dt;
+            }
+        }
+        const double2 sv = s.sv[i];
+        vx = sv.x + vx;
+        vy = sv.y + vy;
+        s.vel[i] = make_double2(vx, vy);
+        const double vm = sqrt(vx * vx + vy * vy) / ni;
+        s.velm[i] = vm;
+        if (!isfinite(vm)) raise_err(err, ERR_NAN);
+    }
+    if (fi & ISPH_FLAG_BUFFER) s.vel[i] = make_double2(s.sv[i].x, 0.0);
+    if ((fi & ISPH_FLAG_DEAD) && !(fi & ISPH_FLAG_BUFFER)) {
+        s.vel[i] = make_double2(0.0, 0.0);
+        s.sv[i] = make_double2(0.0, 0.0);
+    }
+}
+
+// comp_pos, integrator.f90:867-906 (buffer release :888-897 not modelled)
+__global__ void k_comp_pos(int n, StateView s, const SimScalars* ss) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (s.tid[i] == 3 && !(s.flags[i] & ISPH_FLAG_DEAD)) {
+        const double dt = ss->dt;
+        double2 p = s.pos[i], q = s.sp[i], v = s.vel[i];
+        const double nn = s.por[i];
+        p.x = ((p.x + q.x) + (dt * v.x) / nn) / 2.0;
+        p.y = ((p.y + q.y) + (dt * v.y) / nn) / 2.0;
+        s.pos[i] = p;
+    }
+}
+
+// freesurf, isph.f90:49-86, pull form over the symmetric lists
+__global__ void k_freesurf(int n, double thr, size_t ld, StateView s, const int* __restrict__ nbr,
+                           const int* __restrict__ cnt, const double* __restrict__ divr) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    bool v = false;
+    const int c = cnt[i];
+    for (int k = 0; k < c; ++k) {
+        const int j = nbr[(size_t)k * ld + i];
+        if (s.tid[j] == 3 && !(s.flags[j] & ISPH_FLAG_BUFFER) && divr[j] <= thr) v = true;
+    }
+    int fl = s.flags[i];
+    s.flags[i] = v ? (fl | ISPH_FLAG_VICINITY) : (fl & ~ISPH_FLAG_VICINITY);
+}
+
+// ---------------------------------------------------------------------------
+// opt2_shift, part_shift.f90:12-192
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLOCK) k_shift(int n, KernelConsts kc, double csh, double cap,
+                                                 size_t ld, StateView s,
+                                                 const int* __restrict__ nbr,
+                                                 const int* __restrict__ cnt,
+                                                 const double* __restrict__ L) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int fi = s.flags[i];
+    if (s.tid[i] != 3 || (fi & ISPH_FLAG_BUFFER)) return;
+    const int c = cnt[i];
+    if ((fi & ISPH_FLAG_DEAD) || c == 0) {
+        s.sp[i] = make_double2(0.0, 0.0);
+        s.sv[i] = make_double2(0.0, 0.0);
+        return;
+    }
+    double l0, l1, l2, l3;
+    load_L(L, i, l0, l1, l2, l3);
+    const double2 pi = s.pos[i], vi = s.vel[i];
+    const double frac = (fi & ISPH_FLAG_FREE) ? 0.01 : 1.0;
+    const double cf = (csh * (kc.h * kc.h)) * frac;
+    double xs = 0.0, ys = 0.0;
+    double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+    for (int k = 0; k < c; ++k) {
+        const int j = nbr[(size_t)k * ld + i];
+        double r, wx, wy;
+        pair_geom(kc, pi, s.pos[j], r, wx, wy);
+        const double vj = s.mass[j] / s.rho[j];
+        xs = xs - (cf * wx) * vj;
+        ys = ys - (cf * wy) * vj;
+        double gx = l0 * wx + l1 * wy;
+        double gy = l2 * wx + l3 * wy;
+        const double2 vel = s.vel[j];
+        double du = vel.x - vi.x, dv = vel.y - vi.y;
+        t1 = t1 + (gx * du) * vj;
+        t2 = t2 + (gy * du) * vj;
+        t3 = t3 + (gx * dv) * vj;
+        t4 = t4 + (gy * dv) * vj;
+    }
+    double axs = fabs(xs), ays = fabs(ys);
+    double cxs = (xs == 0.0) ? 0.0 : ((axs < cap ? axs : cap) * xs) / axs;
+    double cys = (ys == 0.0) ? 0.0 : ((ays < cap ? ays : cap) * ys) / ays;
+    s.sp[i] = make_double2(cxs, cys);
+    s.sv[i] = make_double2(t1 * xs + t2 * ys, t3 * xs + t4 * ys);
+}
+
+__global__ void k_shift_apply(int n, StateView s) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (s.tid[i] == 3 && !(s.flags[i] & ISPH_FLAG_BUFFER)) {
+        double2 p = s.pos[i], q = s.sp[i], v = s.vel[i], w = s.sv[i];
+        s.pos[i] = make_double2(p.x + q.x, p.y + q.y);
+        s.vel[i] = make_double2(v.x + w.x, v.y + w.y);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// timestep, integrator.f90:908-976: umax = max vel, numax = max |nut|
+// (max is order-independent; non-negative doubles compare as uint64)
+// ---------------------------------------------------------------------------
+__global__ void k_maxes(int n, const double* __restrict__ velm, const double* __restrict__ nut,
+                        unsigned long long* out) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (i < n) {
+        a = fabs(velm[i]);
+        b = fabs(nut[i]);
+        if (!(a == a)) a = 0.0;
+        if (!(b == b)) b = 0.0;
+    }
+    for (int s = 1; s < 64; s <<= 1) {
+        a = fmax(a, __shfl_xor(a, s, 64));
+        b = fmax(b, __shfl_xor(b, s, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], (unsigned long long)__double_as_longlong(a));
+        atomicMax(&out[1], (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+__global__ void k_timestep(const unsigned long long* mx, SimScalars* ss, double dl, double s1,
+                           double mu, double rho, int adaptive) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double umax = __longlong_as_double((long long)mx[0]);
+    const double numax = __longlong_as_double((long long)mx[1]);
+    ss->umax = umax;
+    ss->numax = numax;
+    if (adaptive) {
+        const double t = ss->t;
+        double dt;
+        if (t <= 1.0) {
+            dt = fmin(fmin(s1 * dl / umax, s1 * (dl * dl) / ((mu / rho) + numax)), 0.0010);
+        } else if (t > 1.0 && t <= 2.50) {
+            float a = (float)(s1 * dl / umax), b = (float)(s1 * (dl * dl) * rho / mu);
+            float m = a < b ? a : b;
+            m = m < 0.002f ? m : 0.002f;
+            dt = (double)m;
+        } else if (t > 3.0 && t <= 4.0) {
+            float a = (float)(s1 * dl / umax), b = (float)(s1 * (dl * dl) * rho / mu);
+            float m = a < b ? a : b;
+            m = m < 0.004f ? m : 0.004f;
+            dt = (double)m;
+        } else {
+            dt = fmin(fmin(s1 * dl / umax, s1 * (dl * dl) / ((mu / rho) + numax)), 0.0050);
+        }
+        ss->dt = dt;
+    }
+}
+
+__global__ void k_advance_time(SimScalars* ss) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ss->t = ss->t + ss->dt;
+}
+
+// plain streaming copy (bandwidth and counter calibration)
+__global__ void k_copy(const double2* __restrict__ a, double2* __restrict__ b, size_t n) {
+    size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * BLOCK;
+    for (; i < n; i += stride) b[i] = a[i];
+}
+
+// explicit instantiations
+template __global__ void k_bicg_1<double>(SolveArgsT<double>, SolverScalars*, const double*,
+                                          const double*, double*, double*);
+template __global__ void k_bicg_1<float>(SolveArgsT<float>, SolverScalars*, const float*,
+                                         const float*, float*, float*);
+template __global__ void k_bicg_2<double>(SolveArgsT<double>, SolverScalars*, const double*);
+template __global__ void k_bicg_2<float>(SolveArgsT<float>, SolverScalars*, const float*);
+template __global__ void k_bicg_3<double>(SolveArgsT<double>, SolverScalars*, const double*, int);
+template __global__ void k_bicg_3<float>(SolveArgsT<float>, SolverScalars*, const float*, int);
+
+}  // namespace isph

@@ -1,0 +1,216 @@
+/*
+ * isph_capi.h -- C ABI of the gfx950 ISPH step (libisph.so).
+ *
+ * The reference solver (a Fortran 2008 / OpenMP 2-D ISPH code) keeps its
+ * particle state in module globals and exposes one zero-argument subroutine
+ * per stage of a time step; `main.f90:41-66` calls them in a fixed order.
+ * This header replaces each of those subroutines by one `extern "C"` entry
+ * point that takes an opaque context.  All particle state lives in HBM
+ * inside the context; the Fortran driver (iso_c_binding, see
+ * beach_slope_confined_amd/fortran/isph_capi.f90) and the Python harness
+ * (ctypes) only pass plain pointers and sizes.
+ *
+ * Conventions
+ *   - every function returns an int status: ISPH_OK (0) or a negative
+ *     ISPH_E_* code; isph_last_error() returns a message for the last one.
+ *     Nothing aborts or exits across the ABI.
+ *   - one host thread per context; all work is ordered on the context's own
+ *     HIP stream (default priority) and every entry point is synchronous
+ *     when it returns (isph_step() synchronises once per call).
+ *   - host pointers are borrowed for the duration of the call.
+ *   - particle arrays are structure-of-arrays, 0-based, in the context's
+ *     current order.  After isph_cellshift()/isph_cellshiftalt() that order
+ *     is the cell-sorted order, which is also the PPE row/column order (the
+ *     reference's `matid`, `isph.f90:88-113`).  `pid` travels with the
+ *     particle and identifies it across re-orderings.
+ *   - particle types follow the reference (`particle.f90:92`,
+ *     `boundary.f90:50-83`): tid 1 = fixed wall (innermost layer),
+ *     tid 2 = dummy wall, tid 3 = fluid.  Soil particles (the reference's
+ *     tid 4 `porlist`, `boundary.f90:523-593`) are passed separately with
+ *     isph_set_soil().
+ */
+#ifndef ISPH_CAPI_H
+#define ISPH_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define ISPH_OK            0
+#define ISPH_E_ARG        -1   /* bad argument or size                         */
+#define ISPH_E_HIP        -2   /* HIP runtime error                            */
+#define ISPH_E_CAPACITY   -3   /* neighbour list or particle capacity exceeded */
+#define ISPH_E_NAN        -4   /* non-finite value produced                    */
+#define ISPH_E_NOCONV     -5   /* PPE solve did not converge / broke down      */
+#define ISPH_E_DOMAIN     -6   /* particle outside the cell grid               */
+#define ISPH_E_SINGULAR   -7   /* PPE row with zero or non-finite diagonal     */
+#define ISPH_E_STATE      -8   /* call out of order (e.g. solve before assembly) */
+
+/* ---- enumerations ------------------------------------------------------ */
+#define ISPH_KERNEL_HYBRID       0   /* kernel.f90:52-62,113-124 (alp = 0.1)    */
+#define ISPH_KERNEL_WENDLAND_C2  1   /* kernel.f90:42-43,75,102-103             */
+
+#define ISPH_DRAG_REFERENCE 0  /* porous.f90:192,204-212: lamk = alpha, vis(oden) */
+#define ISPH_DRAG_ERGUN     1  /* R = -alpha nu (1-n)^2/(n^3 d50^2) - beta |v| (1-n)/(n^3 d50) */
+
+#define ISPH_PREC_FP64  0      /* FP64 matrix, FP64 BiCGSTAB                   */
+#define ISPH_PREC_MIXED 1      /* FP32 matrix + FP32 inner BiCGSTAB, FP64 refinement */
+
+/* particle flag bits (particle.f90:123-124) */
+#define ISPH_FLAG_DEAD      1
+#define ISPH_FLAG_BUFFER    2
+#define ISPH_FLAG_FREE      4
+#define ISPH_FLAG_VICINITY  8
+
+/* Run parameters.  The reference keeps these as module globals
+ * (initialize.f90:19-79) and scattered constants (kernel.f90:30,
+ * porous.f90:111, part_shift.f90:18,71, turbulence.f90:16); setup.f90:47-82
+ * derives h1, dl and lam from them.  All doubles come first, then int32s,
+ * so that C, Fortran bind(C) and ctypes agree on the layout. */
+typedef struct isph_params {
+    double h;          /* smoothing length (reference h1)                      */
+    double cell_size;  /* cell edge, >= 2h (reference uses 4h + quadrants)    */
+    double xmin, ymin; /* lower-left corner of the cell grid                   */
+    double xmax, ymax; /* upper-right corner of the cell grid                  */
+    double dl;         /* particle spacing (setup.f90:53)                      */
+    double lam;        /* eta in 1/(r^2+eta), 1e-3 h^2 (setup.f90:82)          */
+    double dt;         /* time step                                            */
+    double rho0;       /* reference density `rho` (initialize.f90:21)          */
+    double rel_den;    /* salt/fresh density ratio (initialize.f90:25)         */
+    double g;          /* signed gravity along y (initialize.f90:21)           */
+    double nu;         /* kinematic viscosity for ISPH_DRAG_ERGUN              */
+    double csh;        /* shifting coefficient (initialize.f90:22)             */
+    double maxshift;   /* shift cap as a fraction of dl (initialize.f90:36)    */
+    double lamfs;      /* free-surface factor (initialize.f90:42)              */
+    double maxdivr;    /* free-surface divergence scale (initialize.f90:41)    */
+    double por;        /* porosity of the soil skeleton (initialize.f90:24)    */
+    double alpha;      /* Darcy coefficient (reference lamk, porous.f90:111)   */
+    double beta;       /* Forchheimer coefficient (ISPH_DRAG_ERGUN only)       */
+    double d50;        /* grain diameter (porous.f90:111)                      */
+    double tol;        /* PPE relative residual tolerance ||b-Ax||/||b||        */
+    double inner_tol;  /* mixed precision: inner FP32 relative tolerance       */
+    double xlcutoff;   /* outflow sink (integrator.f90:60-61), active if sink  */
+    double ytcutoff;
+    double sig1;       /* adaptive dt factor (initialize.f90:35)               */
+    double mu;         /* dynamic viscosity used by adaptive dt (initialize.f90:35) */
+    int32_t kernel;      /* ISPH_KERNEL_*                                       */
+    int32_t drag_law;    /* ISPH_DRAG_*                                         */
+    int32_t precision;   /* ISPH_PREC_*                                         */
+    int32_t maxit;       /* BiCGSTAB iteration cap (FP64, or total inner)       */
+    int32_t inner_maxit; /* mixed precision: inner iterations per refinement    */
+    int32_t max_outer;   /* mixed precision: refinement steps                   */
+    int32_t max_nbr;     /* neighbour-list capacity per particle                */
+    int32_t shifting;    /* 1: opt2_shift moves particles                       */
+    int32_t adaptive_dt; /* 1: isph_timestep() updates dt (integrator.f90:950-960) */
+    int32_t sink;        /* 1: isph_cellshiftalt() deletes outflow particles    */
+    int32_t visc_law;    /* 0: mu = vis(oden) (porous.f90:12-30); 1: mu = nu*oden */
+    int32_t reserved;    /* keeps the struct a multiple of 8 bytes; set to 0    */
+} isph_params;
+
+typedef struct isph_ctx isph_ctx;
+
+/* Fields readable with isph_get_field() (double arrays of n, or 4n for L). */
+#define ISPH_FIELD_XS       0   /* projection's old x, then shift dx          */
+#define ISPH_FIELD_YS       1
+#define ISPH_FIELD_VXS      2   /* intermediate velocity v*, then shift dv     */
+#define ISPH_FIELD_VYS      3
+#define ISPH_FIELD_POROSITY 4   /* n  (porous.f90:188)                          */
+#define ISPH_FIELD_RESIST   5   /* R  (porous.f90:204-212, resistx = resisty)   */
+#define ISPH_FIELD_COFF     6   /* KGC matrix L, 4 per particle (search.f90:276-278) */
+#define ISPH_FIELD_DIVR     7   /* gradvx = A11 + A22 (search.f90:272)          */
+#define ISPH_FIELD_NUT      8   /* eddy viscosity (turbulence.f90:80)           */
+#define ISPH_FIELD_TKE      9   /* turbulent kinetic energy (turbulence.f90:83) */
+#define ISPH_FIELD_VARTS   10   /* free-surface artificial viscosity (integrator.f90:495) */
+#define ISPH_FIELD_VEL     11   /* |v|/n (integrator.f90:825)                   */
+#define ISPH_FIELD_RHS     12   /* Jacobi-scaled PPE right-hand side            */
+
+/* ---- context ------------------------------------------------------------ */
+/* Allocates every device buffer for up to cap_particles particles and
+ * cap_soil soil particles (replaces memory.f90:10-147). */
+int isph_create(const isph_params* prm, int32_t cap_particles, int32_t cap_soil,
+                isph_ctx** out);
+int isph_destroy(isph_ctx* ctx);
+const char* isph_last_error(const isph_ctx* ctx);
+int isph_set_params(isph_ctx* ctx, const isph_params* prm);
+int isph_get_params(const isph_ctx* ctx, isph_params* prm);
+
+/* ---- state transfer ----------------------------------------------------- */
+/* Uploads particles (replaces gen_boundary/gen_fluid's placement into
+ * dpcell, boundary.f90:12, internal.f90:10) and bins them into cells.
+ * flags may be NULL (all zero).  Sets vel = |v|, umax = 0, t = 0. */
+int isph_set_particles(isph_ctx* ctx, int32_t n,
+                       const double* x, const double* y,
+                       const double* vx, const double* vy,
+                       const double* mass, const double* density,
+                       const double* oden, const double* pressure,
+                       const int32_t* tid, const int32_t* flags,
+                       const int32_t* pid);
+/* Downloads particles in the current order; any pointer may be NULL. */
+int isph_get_particles(isph_ctx* ctx, int32_t* n,
+                       double* x, double* y, double* vx, double* vy,
+                       double* mass, double* density, double* oden,
+                       double* pressure, int32_t* tid, int32_t* flags,
+                       int32_t* pid);
+/* Soil skeleton particles with mass and density (porlist,
+ * boundary.f90:569-588); porsearch uses m/rho (porous.f90:54-55). */
+int isph_set_soil(isph_ctx* ctx, int32_t ns, const double* x, const double* y,
+                  const double* mass, const double* density);
+int isph_get_field(isph_ctx* ctx, int32_t field, double* out);
+int isph_set_field(isph_ctx* ctx, int32_t field, const double* in);
+/* Neighbour lists of the last isph_neighbour() as CSR over the current
+ * order: ptr[n+1], idx[ptr[n]] ascending per row (the verlet list,
+ * interactions.f90:26-35).  Pass idx = NULL to read ptr only. */
+int isph_get_neighbours(isph_ctx* ctx, int32_t* ptr, int32_t* idx);
+/* PPE matrix of the last assembly as CSR with the diagonal in place
+ * (fmatrix/fvec after Jacobi scaling, isph.f90:423-442): rowptr[n+1],
+ * col[nnz] strictly ascending per row, val[nnz], rhs[n].  Any of col, val,
+ * rhs may be NULL. */
+int isph_get_csr(isph_ctx* ctx, int32_t* rowptr, int32_t* col, double* val,
+                 double* rhs);
+/* Scalars: umax, dt, t, and the last solve's statistics. */
+int isph_get_scalars(isph_ctx* ctx, double* umax, double* dt, double* t,
+                     int32_t* iters, int32_t* outer, double* relres);
+int isph_set_scalars(isph_ctx* ctx, double umax, double dt, double t);
+
+/* ---- one entry per reference subroutine (main.f90:41-66 order) --------- */
+int isph_projection(isph_ctx* ctx);    /* isph.f90:16-47                     */
+int isph_cellshift(isph_ctx* ctx);     /* integrator.f90:250-391             */
+int isph_resetid(isph_ctx* ctx);       /* isph.f90:88-113                    */
+int isph_neighbour(isph_ctx* ctx);     /* search.f90:310-399 + nsearch2 :13-308 */
+int isph_effpor(isph_ctx* ctx);        /* porous.f90:107-224 (+ porsearch :32) */
+int isph_eddyvis(isph_ctx* ctx);       /* turbulence.f90:12-110              */
+int isph_int_vel(isph_ctx* ctx);       /* integrator.f90:458-713             */
+/* ppesolve = assembly (isph.f90:123-456) + solve (:460) + scatter (:465-483) */
+int isph_ppe_assemble(isph_ctx* ctx);
+int isph_ppe_solve(isph_ctx* ctx, int32_t* iters, double* relres);
+int isph_ppesolve(isph_ctx* ctx, int32_t* iters, double* relres);
+int isph_comp_vel(isph_ctx* ctx);      /* integrator.f90:715-865             */
+int isph_comp_pos(isph_ctx* ctx);      /* integrator.f90:867-906             */
+int isph_cellshiftalt(isph_ctx* ctx);  /* integrator.f90:32-248              */
+int isph_freesurf(isph_ctx* ctx);      /* isph.f90:49-86                     */
+int isph_opt2_shift(isph_ctx* ctx);    /* part_shift.f90:12-192              */
+int isph_timestep(isph_ctx* ctx);      /* integrator.f90:908-976             */
+/* nsteps whole steps in the main.f90:41-66 order (no host round trip
+ * between stages). */
+int isph_step(isph_ctx* ctx, int32_t nsteps);
+
+/* ---- instrumentation (no reference counterpart) -------------------------- */
+/* When enabled, HIP events bracket every launch of the BiCGSTAB SpMV kernels
+ * on the context stream; isph_get_kernel_time() returns launches and total
+ * milliseconds since the last reset.  kernel: 0 = FP64 p-update+SpMV,
+ * 1 = FP64 s+SpMV, 2 = FP32 p-update+SpMV, 3 = FP32 s+SpMV. */
+int isph_profile_enable(isph_ctx* ctx, int32_t enable);
+int isph_get_kernel_time(isph_ctx* ctx, int32_t kernel, int64_t* launches,
+                         double* total_ms);
+/* Copies `bytes` bytes device-to-device with the library's copy kernel
+ * `reps` times on the context stream; returns the mean ms per copy.  Used to
+ * calibrate attainable HBM bandwidth and the FETCH_SIZE/WRITE_SIZE counters. */
+int isph_copy_bench(isph_ctx* ctx, int64_t bytes, int32_t reps, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ISPH_CAPI_H */
