@@ -86,6 +86,10 @@ def load_library(path=LIB_PATH):
     """Load libisph.so.  There is no fallback: a missing library is an error."""
     global _LIB
     if _LIB is None:
+        # torch ships its own libamdhip64.so.7 (same soname as /opt/rocm's);
+        # loading torch first makes the process use one HIP runtime, shared
+        # by torch and this library.
+        import torch  # noqa: F401
         if not os.path.exists(path):
             raise ImportError(
                 f"{path} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
@@ -97,6 +101,8 @@ def load_library(path=LIB_PATH):
             ("get_kernel_time", [ctypes.c_void_p, ctypes.c_int32,
                                  ctypes.POINTER(ctypes.c_int64), _c_dp], ctypes.c_int),
             ("copy_bench", [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, _c_dp], ctypes.c_int),
+            ("checkpoint", [ctypes.c_void_p, ctypes.c_int32], ctypes.c_int),
+            ("restore", [ctypes.c_void_p, ctypes.c_int32], ctypes.c_int),
         ]:
             f = getattr(lib, "isph_" + name)
             f.argtypes = args
@@ -283,6 +289,13 @@ class IsphSim(Sim):
 
     def __init__(self, params, cap, cap_soil=0, lib=None):
         super().__init__(lib or load_library(), "isph_", params, cap, cap_soil)
+
+    def checkpoint(self, slot):
+        self._check(self._lib.isph_checkpoint(self._h, int(slot)), "checkpoint")
+
+    def restore(self, slot):
+        self._check(self._lib.isph_restore(self._h, int(slot)), "restore")
+        self.n = self.count()
 
     def profile_enable(self, on=True):
         self._check(self._lib.isph_profile_enable(self._h, 1 if on else 0), "profile_enable")

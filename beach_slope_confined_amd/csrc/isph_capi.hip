@@ -57,7 +57,10 @@ struct isph_ctx {
     double *val64 = nullptr, *rhs = nullptr;
     float* val32 = nullptr;
     int* dpos = nullptr;
+    short* nbr16 = nullptr;
     bool assembled = false;
+    bool assembled32 = false;
+    bool narrow_idx = false;
     double* v64[12] = {};
     float* v32[10] = {};
     double* part = nullptr;
@@ -65,7 +68,6 @@ struct isph_ctx {
     SolverScalars* sc = nullptr;
     SolverScalars* sc_host = nullptr;  // pinned
     SimScalars* ss = nullptr;
-    unsigned long long* maxbuf = nullptr;
     uint32_t* err = nullptr;
     double2* spos = nullptr;
     double *smass = nullptr, *srho = nullptr;
@@ -73,6 +75,13 @@ struct isph_ctx {
     int iters = 0, outer = 0;
     double relres = 0.0;
     Profiler prof;
+    struct Slot {
+        StateView v{};
+        int* key = nullptr;
+        SimScalars ss{};
+        int n = 0;
+        bool used = false;
+    } slots[64];
     std::string msg;
 };
 
@@ -220,6 +229,11 @@ int sort_particles(isph_ctx* c, bool with_sink) {
 
 KernelConsts kconsts(const isph_ctx* c) { return make_kc(c->prm); }
 
+__global__ void k_fill(int n, double* x, double v) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
 // ---- stage launchers (enqueue only) ----------------------------------------
 int run_projection(isph_ctx* c) {
     if (c->n) k_projection<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->st[c->cur], c->ss);
@@ -277,10 +291,13 @@ int run_assemble(isph_ctx* c) {
     if (c->n)
         k_assemble<<<nblk(c->n), BLOCK, 0, c->stream>>>(
             c->n, kconsts(c), p.lam, p.rho0 * p.rel_den, (size_t)c->cap, c->st[c->cur], c->nbr,
-            c->cnt, c->L, c->val64, p.precision == ISPH_PREC_MIXED ? c->val32 : nullptr, c->dpos,
+            c->cnt, c->L, c->val64,
+            (p.precision == ISPH_PREC_MIXED || p.precision == ISPH_PREC_MIXED_V64) ? c->val32 : nullptr,
+            c->dpos,
             c->rhs, c->ss, c->err);
     HIPCHK(c, hipGetLastError());
     c->assembled = true;
+    c->assembled32 = p.precision == ISPH_PREC_MIXED || p.precision == ISPH_PREC_MIXED_V64;
     return ISPH_OK;
 }
 
@@ -322,33 +339,20 @@ int run_shift(isph_ctx* c) {
 }
 
 int run_timestep(isph_ctx* c) {
-    HIPCHK(c, hipMemsetAsync(c->maxbuf, 0, 2 * sizeof(unsigned long long), c->stream));
+    const int nb = std::max(nblk(c->n), 1);
     if (c->n)
         k_maxes<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->st[c->cur].velm, c->st[c->cur].nut,
-                                                      c->maxbuf);
+                                                      c->part);
+    else
+        k_fill<<<1, BLOCK, 0, c->stream>>>(2, c->part, 0.0);
     const isph_params& p = c->prm;
-    k_timestep<<<1, 64, 0, c->stream>>>(c->maxbuf, c->ss, p.dl, p.sig1, p.mu, p.rho0,
-                                        p.adaptive_dt);
+    k_timestep<<<1, BLOCK, 0, c->stream>>>(nb, c->part, c->ss, p.dl, p.sig1, p.mu, p.rho0,
+                                           p.adaptive_dt);
     HIPCHK(c, hipGetLastError());
     return ISPH_OK;
 }
 
 // ---- PPE solve -------------------------------------------------------------
-__global__ void k_reset_solver(SolverScalars* sc) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        sc->iters = 0;
-        sc->total = 0;
-        sc->done = 0;
-        sc->status = 0;
-        sc->first = 1;
-        sc->relres = 0.0;
-    }
-}
-
-__global__ void k_fill(int n, double* x, double v) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) x[i] = v;
-}
 
 int read_sc(isph_ctx* c) {
     HIPCHK(c, hipMemcpyAsync(c->sc_host, c->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost,
@@ -404,30 +408,18 @@ void prof_collect(isph_ctx* c) {
     }
 }
 
-template <class T>
-SolveArgsT<T> targs(isph_ctx* c, const T* val, T* const* v, double tol) {
-    SolveArgsT<T> a;
-    a.n = c->n;
-    a.ld = (size_t)c->cap;
-    a.nbr = c->nbr;
-    a.cnt = c->cnt;
-    a.dpos = c->dpos;
-    a.val = val;
-    a.rs = v[0];
-    a.rhat = v[1];
-    a.si = v[6];
-    a.mm2 = v[7];
-    a.xs = v[8];
-    a.part = c->part;
-    a.counters = c->counters;
-    a.tol = tol;
-    a.maxit = c->prm.maxit;
-    a.inner_maxit = c->prm.inner_maxit;
-    return a;
+SolveCtl solve_ctl(const isph_ctx* c, int mixed) {
+    SolveCtl t;
+    t.tol = c->prm.tol;
+    t.delta = c->prm.inner_tol;
+    t.maxit = c->prm.maxit;
+    t.maxupd = c->prm.max_outer;
+    t.mixed = mixed;
+    return t;
 }
 
-SolveArgs sargs(isph_ctx* c) {
-    SolveArgs a;
+ResidArgs resid_args(isph_ctx* c, double* x, double* r) {
+    ResidArgs a;
     a.n = c->n;
     a.ld = (size_t)c->cap;
     a.nbr = c->nbr;
@@ -435,120 +427,191 @@ SolveArgs sargs(isph_ctx* c) {
     a.dpos = c->dpos;
     a.val = c->val64;
     a.b = c->rhs;
-    a.x = c->v64[8];
-    a.rs = c->v64[0];
-    a.rhat = c->v64[1];
+    a.x = x;
+    a.r = r;
     a.part = c->part;
-    a.counters = c->counters;
-    a.tol = c->prm.tol;
-    a.maxit = c->prm.maxit;
     return a;
 }
 
-// Run BiCGSTAB iterations in batches; the device sets sc->done, the host
-// polls it after each batch.  Vectors v: 0 rs, 1 rhat, 2/3 p ping-pong,
-// 4/5 mm1 ping-pong, 6 si, 7 mm2, 8 x.  prof slot k0 (SpMV kernels).
-template <class T>
-int iterate(isph_ctx* c, const SolveArgsT<T>& a, T* const* v, int inner, int k0) {
+template <class TV, class TX, class TI>
+KrylovArgs<TV, TX, TI> krylov_args(isph_ctx* c, const TI* nbr, const TV* val, TX* const* v) {
+    KrylovArgs<TV, TX, TI> a;
+    a.n = c->n;
+    a.ld = (size_t)c->cap;
+    a.nbr = nbr;
+    a.cnt = c->cnt;
+    a.dpos = c->dpos;
+    a.val = val;
+    a.rs = v[0];
+    a.rhat = v[1];
+    a.p = v[2];
+    a.q = v[3];
+    a.mm1 = v[4];
+    a.si = v[5];
+    a.mm2 = v[6];
+    a.xs = v[7];
+    a.part = c->part;
+    return a;
+}
+
+// BiCGSTAB iterations in batches until the device sets sc->done; the host
+// polls after each batch.  `first` selects p = r for the first iteration
+// after a (re)start.  prof slots k0, k0+1: the two SpMV kernels.
+template <class TV, class TX, class TI>
+int iterate(isph_ctx* c, const KrylovArgs<TV, TX, TI>& a, const SolveCtl& ctl, bool& first,
+            int k0) {
     const int grid = nblk8(c->n);
-    int cidx = 0;
     for (;;) {
-        int before = c->sc_host->iters;
+        const int before = c->sc_host->total;
         int batch = 8;
-        if (c->sc_host->nb > 0 && c->sc_host->rr > 0) {
-            double rel = std::sqrt(c->sc_host->rr) / (inner ? std::sqrt(c->sc_host->rr0) : c->sc_host->nb);
-            batch = rel > 1e3 * a.tol ? 32 : (rel > 10.0 * a.tol ? 16 : 4);
+        const SolverScalars& h = *c->sc_host;
+        if (h.nb > 0 && h.rr > 0) {
+            double target = ctl.tol * h.nb;
+            if (ctl.mixed) target = std::max(target, ctl.delta * h.rupd);
+            const double ratio = std::sqrt(h.rr) / target;
+            batch = ratio > 100.0 ? 32 : (ratio > 3.0 ? 16 : 4);
         }
         for (int b = 0; b < batch; ++b) {
             prof_record(c, k0, true);
-            k_bicg_1<T><<<grid, BLOCK, 0, c->stream>>>(a, c->sc, v[2 + cidx], v[4 + cidx],
-                                                        v[3 - cidx], v[5 - cidx]);
+            if (first)
+                k_krylov_1t<TV, TX, TI, true><<<grid, BLOCK, 0, c->stream>>>(a, c->sc);
+            else
+                k_krylov_1t<TV, TX, TI, false><<<grid, BLOCK, 0, c->stream>>>(a, c->sc);
             prof_record(c, k0, false);
+            k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_ALPHA, grid, c->part, c->sc, ctl);
             prof_record(c, k0 + 1, true);
-            k_bicg_2<T><<<grid, BLOCK, 0, c->stream>>>(a, c->sc, v[5 - cidx]);
+            k_krylov_2<TV, TX, TI><<<grid, BLOCK, 0, c->stream>>>(a, c->sc);
             prof_record(c, k0 + 1, false);
-            k_bicg_3<T><<<grid, BLOCK, 0, c->stream>>>(a, c->sc, v[3 - cidx], inner);
-            cidx ^= 1;
+            k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_OMEGA, grid, c->part, c->sc, ctl);
+            k_krylov_3<TV, TX, TI><<<grid, BLOCK, 0, c->stream>>>(a, c->sc);
+            k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_BETA, grid, c->part, c->sc, ctl);
+            first = false;
         }
         HIPCHK(c, hipGetLastError());
         int rc = read_sc(c);
         if (rc) return rc;
-        prof_mark(c, k0, c->sc_host->iters - before, batch);
+        prof_mark(c, k0, c->sc_host->total - before, batch);
         if (c->sc_host->done) return ISPH_OK;
     }
 }
 
 int solve_fp64(isph_ctx* c) {
     const int grid = nblk8(c->n);
-    SolveArgs a = sargs(c);
     double* const* v = c->v64;
-    SolveArgsT<double> ta = targs<double>(c, c->val64, v, c->prm.tol);
-    k_reset_solver<<<1, 64, 0, c->stream>>>(c->sc);
-    k_bicg_init<<<grid, BLOCK, 0, c->stream>>>(a, c->sc);
+    double* x = v[8];
+    // the FP64 solve builds x in place: slot 7 (segment) aliases x
+    double* vv[8] = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], x};
+    auto ka = krylov_args<double, double, int>(c, c->nbr, c->val64, vv);
+    const SolveCtl ctl = solve_ctl(c, 0);
+    ResidArgs ra = resid_args(c, x, v[9]);
+    k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 1, v[0], v[1]);
+    k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, ctl);
     HIPCHK(c, hipGetLastError());
     int rc = read_sc(c);
     if (rc) return rc;
+    bool first = true;
     for (;;) {
-        int d = c->sc_host->done;
+        const int d = c->sc_host->done;
         if (d == DONE_ZERO_RHS) {
-            k_fill<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, a.x, 0.0);
+            k_fill<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, x, 0.0);
             return ISPH_OK;
         }
         if (d == DONE_CONVERGED) return ISPH_OK;
         if (d == DONE_FAILED) return fail(c, ISPH_E_NOCONV, "PPE solve did not converge");
         if (d == DONE_RECURRENCE) {
-            k_bicg_true<<<grid, BLOCK, 0, c->stream>>>(a, c->sc);
+            k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 1, v[0], v[1]);
+            k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_TRUE, grid, c->part, c->sc, ctl);
             HIPCHK(c, hipGetLastError());
             if ((rc = read_sc(c))) return rc;
+            first = true;
             continue;
         }
-        if ((rc = iterate<double>(c, ta, v, 0, 0))) return rc;
+        if ((rc = iterate(c, ka, ctl, first, 0))) return rc;
+    }
+}
+
+template <class TX, class TI>
+int solve_mixed_t(isph_ctx* c, const TI* nbr, TX* const* v, int k0) {
+    const int grid = nblk8(c->n);
+    double* x = c->v64[8];
+    double* r64 = c->v64[9];
+    auto ka = krylov_args<float, TX, TI>(c, nbr, c->val32, v);
+    SolveCtl ctl = solve_ctl(c, 1);
+    SolveCtl ctl_flush = solve_ctl(c, 2);
+    ResidArgs ra = resid_args(c, x, r64);
+    k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 0, nullptr, nullptr);
+    k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, solve_ctl(c, 0));
+    k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_RESID, grid, c->part, c->sc, ctl);
+    HIPCHK(c, hipGetLastError());
+    int rc = read_sc(c);
+    if (rc) return rc;
+    if (c->sc_host->done == DONE_ZERO_RHS) {
+        k_fill<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, x, 0.0);
+        return ISPH_OK;
+    }
+    if (c->sc_host->done == DONE_CONVERGED) return ISPH_OK;
+    k_mx_start<TX><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, v[0], v[1], v[7], c->part, c->sc);
+    k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_START, grid, c->part, c->sc, ctl);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = read_sc(c))) return rc;
+    bool first = true;
+    for (;;) {
+        if ((rc = iterate(c, ka, ctl, first, k0))) return rc;
+        const int d = c->sc_host->done;
+        if (d == DONE_FAILED) return fail(c, ISPH_E_NOCONV, "mixed-precision PPE solve did not converge");
+        if (d != DONE_UPDATE) return fail(c, ISPH_E_STATE, "unexpected solver state");
+        // reliable update: flush x, FP64 true residual, replace r
+        k_mx_flush<TX><<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, x, v[7]);
+        k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 0, nullptr, nullptr);
+        k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_RESID, grid, c->part, c->sc, ctl_flush);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = read_sc(c))) return rc;
+        if (c->sc_host->done == DONE_CONVERGED) return ISPH_OK;
+        if (c->sc_host->done == DONE_FAILED)
+            return fail(c, ISPH_E_NOCONV, "mixed-precision PPE solve did not converge");
+        k_mx_replace<TX><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, v[0], v[1], c->part, c->sc);
+        k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_REPLACE, grid, c->part, c->sc, ctl);
+        HIPCHK(c, hipGetLastError());
     }
 }
 
 int solve_mixed(isph_ctx* c) {
-    const int grid = nblk8(c->n);
-    SolveArgs a = sargs(c);
-    float* const* v = c->v32;
-    SolveArgsT<float> fa = targs<float>(c, c->val32, v, c->prm.inner_tol);
-    double* r64 = c->v64[9];
-    k_reset_solver<<<1, 64, 0, c->stream>>>(c->sc);
-    c->outer = 0;
-    int rc;
-    for (;;) {
-        k_mixed_resid<<<grid, BLOCK, 0, c->stream>>>(a, c->sc, r64);
-        HIPCHK(c, hipGetLastError());
-        if ((rc = read_sc(c))) return rc;
-        int d = c->sc_host->done;
-        if (d == DONE_ZERO_RHS) {
-            k_fill<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, a.x, 0.0);
-            return ISPH_OK;
-        }
-        if (d == DONE_CONVERGED) return ISPH_OK;
-        if (c->outer >= c->prm.max_outer || c->sc_host->total >= c->prm.maxit)
-            return fail(c, ISPH_E_NOCONV, "mixed-precision PPE solve did not converge");
-        c->outer++;
-        k_mixed_start<<<grid, BLOCK, 0, c->stream>>>(fa, c->sc, r64);
-        HIPCHK(c, hipGetLastError());
-        if ((rc = read_sc(c))) return rc;
-        if ((rc = iterate<float>(c, fa, v, 1, 2))) return rc;
-        k_mixed_update<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, a.x, v[8], c->sc);
+    // 16-bit column offsets when every offset fits, else the int32 columns
+    k_to_short<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, (size_t)c->cap, c->nbr, c->cnt, c->nbr16,
+                                                     c->err);
+    uint32_t e = 0;
+    HIPCHK(c, hipMemcpyAsync(&e, c->err, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const bool narrow = !(e & ERR_INDEX16);
+    if (!narrow) HIPCHK(c, hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->stream));
+    c->narrow_idx = narrow;
+    if (c->prm.precision == ISPH_PREC_MIXED_V64) {
+        double* vv[8] = {c->v64[0], c->v64[1], c->v64[2], c->v64[3],
+                         c->v64[4], c->v64[5], c->v64[6], c->v64[7]};
+        return narrow ? solve_mixed_t<double, short>(c, c->nbr16, vv, 2)
+                      : solve_mixed_t<double, int>(c, c->nbr, vv, 2);
     }
+    float* vv[8] = {c->v32[0], c->v32[1], c->v32[2], c->v32[3],
+                    c->v32[4], c->v32[5], c->v32[6], c->v32[7]};
+    return narrow ? solve_mixed_t<float, short>(c, c->nbr16, vv, 2)
+                  : solve_mixed_t<float, int>(c, c->nbr, vv, 2);
 }
 
 int run_solve(isph_ctx* c) {
     if (!c->assembled) return fail(c, ISPH_E_STATE, "solve before assembly");
-    if (c->prm.precision == ISPH_PREC_MIXED && !c->val32)
-        return fail(c, ISPH_E_STATE, "mixed precision needs an assembly with precision=MIXED");
+    const bool mixed = c->prm.precision == ISPH_PREC_MIXED || c->prm.precision == ISPH_PREC_MIXED_V64;
+    if (mixed && !c->assembled32)
+        return fail(c, ISPH_E_STATE, "mixed precision needs an assembly with a mixed precision mode");
     StateView& s = c->st[c->cur];
     double* x = c->v64[8];
     if (c->n == 0) return ISPH_OK;
     HIPCHK(c, hipMemcpyAsync(x, s.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
-    int rc = c->prm.precision == ISPH_PREC_MIXED ? solve_mixed(c) : solve_fp64(c);
-    if (rc == ISPH_OK) rc = read_sc(c);
+    int rc = mixed ? solve_mixed(c) : solve_fp64(c);
+    int rc2 = read_sc(c);
+    if (rc == ISPH_OK) rc = rc2;
     c->iters = c->sc_host->total;
     c->relres = c->sc_host->relres;
-    if (c->prm.precision != ISPH_PREC_MIXED) c->outer = 0;
+    c->outer = mixed ? c->sc_host->nupd : 0;
     if (rc != ISPH_OK) return rc;
     // pressure scatter (isph.f90:465-483): rows are particles
     HIPCHK(c, hipMemcpyAsync(s.p, x, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
@@ -619,6 +682,7 @@ int isph_create(const isph_params* prm, int32_t cap, int32_t cap_soil, isph_ctx*
     rc |= dalloc(c, &c->val64, N * M);
     rc |= dalloc(c, &c->val32, N * M);
     rc |= dalloc(c, &c->dpos, N);
+    rc |= dalloc(c, &c->nbr16, N * M);
     rc |= dalloc(c, &c->rhs, N);
     for (int k = 0; k < 10; ++k) rc |= dalloc(c, &c->v64[k], N);
     for (int k = 0; k < 9; ++k) rc |= dalloc(c, &c->v32[k], N);
@@ -626,7 +690,6 @@ int isph_create(const isph_params* prm, int32_t cap, int32_t cap_soil, isph_ctx*
     rc |= dalloc(c, &c->counters, 16);
     rc |= dalloc(c, &c->sc, 1);
     rc |= dalloc(c, &c->ss, 1);
-    rc |= dalloc(c, &c->maxbuf, 2);
     rc |= dalloc(c, &c->err, 1);
     rc |= dalloc(c, &c->spos, (size_t)std::max(cap_soil, 1));
     rc |= dalloc(c, &c->smass, (size_t)std::max(cap_soil, 1));
@@ -654,9 +717,9 @@ int isph_destroy(isph_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_state(c->st[0]);
     free_state(c->st[1]);
-    void* p[] = {c->L, c->divr, c->key, c->key_s, c->rank, c->perm, c->bincnt, c->scanbuf,
+    void* p[] = {c->L, c->divr, c->key, c->key_s, c->rank, c->perm, c->bincnt, c->scanbuf, c->nbr16,
                  c->nbr, c->cnt, c->val64, c->val32, c->dpos, c->rhs, c->part, c->counters,
-                 c->sc, c->ss, c->maxbuf, c->err, c->spos, c->smass, c->srho, c->sstart};
+                 c->sc, c->ss, c->err, c->spos, c->smass, c->srho, c->sstart};
     for (auto q : p)
         if (q) (void)hipFree(q);
     for (int k = 0; k < 12; ++k)
@@ -664,6 +727,11 @@ int isph_destroy(isph_ctx* c) {
     for (int k = 0; k < 10; ++k)
         if (c->v32[k]) (void)hipFree(c->v32[k]);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
+    for (auto& sl : c->slots) {
+        if (!sl.v.pos) continue;
+        free_state(sl.v);
+        if (sl.key) (void)hipFree(sl.key);
+    }
     for (int k = 0; k < 4; ++k)
         for (auto& e : c->prof.pool[k]) {
             (void)hipEventDestroy(e.a);
@@ -1031,6 +1099,54 @@ int isph_step(isph_ctx* c, int32_t nsteps) {
     }
     prof_collect(c);
     return sync_check(c);
+}
+
+static int copy_state(isph_ctx* c, const StateView& a, const StateView& b, int n) {
+    auto D2D = hipMemcpyDeviceToDevice;
+    const size_t n2 = sizeof(double2) * n, nd = sizeof(double) * n, ni = sizeof(int) * n;
+    HIPCHK(c, hipMemcpyAsync(b.pos, a.pos, n2, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.vel, a.vel, n2, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.sp, a.sp, n2, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.sv, a.sv, n2, D2D, c->stream));
+    const double* da[] = {a.mass, a.rho, a.oden, a.p, a.por, a.R, a.nut, a.tke, a.varts, a.velm};
+    double* db[] = {b.mass, b.rho, b.oden, b.p, b.por, b.R, b.nut, b.tke, b.varts, b.velm};
+    for (int k = 0; k < 10; ++k) HIPCHK(c, hipMemcpyAsync(db[k], da[k], nd, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.tid, a.tid, ni, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.flags, a.flags, ni, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.pid, a.pid, ni, D2D, c->stream));
+    return ISPH_OK;
+}
+
+int isph_checkpoint(isph_ctx* c, int32_t slot) {
+    if (!c || slot < 0 || slot >= 64) return ISPH_E_ARG;
+    auto& sl = c->slots[slot];
+    if (!sl.v.pos) {
+        if (alloc_state(c, sl.v, (size_t)c->cap)) return fail(c, ISPH_E_HIP, "checkpoint allocation failed");
+        HIPCHK(c, hipMalloc((void**)&sl.key, sizeof(int) * c->cap));
+    }
+    int rc = copy_state(c, c->st[c->cur], sl.v, c->n);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(sl.key, c->key_s, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&sl.ss, c->ss, sizeof(SimScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    sl.n = c->n;
+    sl.used = true;
+    return ISPH_OK;
+}
+
+int isph_restore(isph_ctx* c, int32_t slot) {
+    if (!c || slot < 0 || slot >= 64) return ISPH_E_ARG;
+    auto& sl = c->slots[slot];
+    if (!sl.used) return fail(c, ISPH_E_STATE, "restore from an empty checkpoint slot");
+    int rc = copy_state(c, sl.v, c->st[c->cur], sl.n);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->key_s, sl.key, sizeof(int) * sl.n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->ss, &sl.ss, sizeof(SimScalars), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n = sl.n;
+    c->have_nbr = false;
+    c->assembled = false;
+    return ISPH_OK;
 }
 
 int isph_profile_enable(isph_ctx* c, int32_t enable) {

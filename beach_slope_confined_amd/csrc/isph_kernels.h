@@ -16,6 +16,7 @@ constexpr uint32_t ERR_CAPACITY = 2u;
 constexpr uint32_t ERR_NAN = 4u;
 constexpr uint32_t ERR_SINGULAR = 8u;
 constexpr uint32_t ERR_NOCONV = 16u;
+constexpr uint32_t ERR_INDEX16 = 32u;  // a column offset does not fit 16 bits
 
 // solver `done` states
 constexpr int DONE_RUNNING = 0;
@@ -23,7 +24,7 @@ constexpr int DONE_RECURRENCE = 1;  // recurrence residual passed, true residual
 constexpr int DONE_CONVERGED = 2;
 constexpr int DONE_ZERO_RHS = 3;
 constexpr int DONE_FAILED = 4;
-constexpr int DONE_INNER = 5;       // mixed precision inner solve finished
+constexpr int DONE_UPDATE = 5;      // mixed precision: reliable update requested
 
 struct GridInfo {
     double xmin, ymin, cs;
@@ -49,11 +50,64 @@ struct SimScalars {
 };
 
 struct SolverScalars {
-    double bb, nb, prevr, alpha, wi, betas, rr, rr0, nr, relres;
-    int iters, total, first, done, status, pad;
+    double bb, nb, prevr, prevr_prev, alpha, wi, betas, rr, rupd, nr, relres;
+    int iters, total, first, done, status, nupd;
 };
 
-struct SolveArgs {
+// ---------------------------------------------------------------------------
+// fixed-tree reductions: wave64 butterfly (xor 1,2,..,32), then the four
+// waves of a 256-thread block as (w0+w1)+(w2+w3)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) v = v + __shfl_xor(v, s, 64);
+    return v;
+}
+
+// block sum in the fixed tree; the result is valid in every thread
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red[4];
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__device__ __forceinline__ double block_max(double v) {
+    __shared__ double red[4];
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) v = fmax(v, __shfl_xor(v, s, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// Logical block of a solver launch: physical blocks are dealt round-robin
+// to the 8 XCDs, so XCD x gets the contiguous logical range
+// [x*per, (x+1)*per) and its L2 holds the vector entries its rows gather.
+__device__ __forceinline__ int xcd_block(int bid, int nblocks) {
+    int per = nblocks >> 3;  // nblocks is a multiple of 8
+    return (bid & 7) * per + (bid >> 3);
+}
+
+// Operands of one BiCGSTAB flavour: TV matrix values, TX vectors, TI column
+// index (int: absolute, short: offset from the row).
+template <class TV, class TX, class TI>
+struct KrylovArgs {
+    int n;
+    size_t ld;
+    const TI* nbr;
+    const int* cnt;
+    const int* dpos;
+    const TV* val;
+    TX *rs, *rhat, *p, *q, *mm1, *si, *mm2, *xs;
+    double* part;  // 2 * nbp block partials
+};
+
+// Operands of an FP64 residual b - A64 x.
+struct ResidArgs {
     int n;
     size_t ld;
     const int* nbr;
@@ -61,34 +115,28 @@ struct SolveArgs {
     const int* dpos;
     const double* val;
     const double* b;
-    double* x;
-    double* rs;
-    double* rhat;
+    const double* x;
+    double* r;
     double* part;
-    uint32_t* counters;
-    double tol;
-    int maxit;
 };
 
-template <class T>
-struct SolveArgsT {
-    int n;
-    size_t ld;
-    const int* nbr;
-    const int* cnt;
-    const int* dpos;
-    const T* val;
-    T* rs;
-    T* rhat;
-    T* si;
-    T* mm2;
-    T* xs;  // the solution being built
-    double* part;
-    uint32_t* counters;
-    double tol;
-    int maxit;
-    int inner_maxit;
+struct SolveCtl {
+    double tol;    // ||b - A x|| <= tol ||b||
+    double delta;  // reliable update when the residual fell by delta
+    int maxit;     // iteration cap (total)
+    int maxupd;    // reliable update cap
+    int mixed;     // 1: reliable-update logic, 0: plain FP64 BiCGSTAB
 };
+
+// reductions finished by one block (k_finish)
+constexpr int FIN_INIT = 0;     // (b,b), (r,r) of the FP64 start
+constexpr int FIN_ALPHA = 1;    // (v,rhat)
+constexpr int FIN_OMEGA = 2;    // (t,s), (t,t)
+constexpr int FIN_BETA = 3;     // (r,rhat), (r,r) + stop tests
+constexpr int FIN_TRUE = 4;     // (r,r) of the FP64 true residual
+constexpr int FIN_RESID = 5;    // (b,b), (r,r) of the mixed outer residual
+constexpr int FIN_START = 6;    // (rhat,r) of the mixed start
+constexpr int FIN_REPLACE = 7;  // (r,rhat) after a reliable update
 
 __global__ void k_bin_keys(int n, GridInfo g, const double2* pos, const int* tid, const int* flags,
                            int sink, double xlcut, double ytcut, int* key, int* bincnt, int* rank,
@@ -118,18 +166,24 @@ __global__ void k_assemble(int n, KernelConsts kc, double lam, double lampd, siz
                            StateView s, const int* nbr, const int* cnt, const double* L,
                            double* val, float* val32, int* dpos, double* rhs,
                            const SimScalars* ss, uint32_t* err);
-__global__ void k_bicg_init(SolveArgs a, SolverScalars* sc);
-template <class T>
-__global__ void k_bicg_1(SolveArgsT<T> a, SolverScalars* sc, const T* p_old, const T* mm1_old,
-                         T* p_new, T* mm1_new);
-template <class T>
-__global__ void k_bicg_2(SolveArgsT<T> a, SolverScalars* sc, const T* mm1);
-template <class T>
-__global__ void k_bicg_3(SolveArgsT<T> a, SolverScalars* sc, const T* p, int inner);
-__global__ void k_bicg_true(SolveArgs a, SolverScalars* sc);
-__global__ void k_mixed_resid(SolveArgs a, SolverScalars* sc, double* r64);
-__global__ void k_mixed_start(SolveArgsT<float> f, SolverScalars* sc, const double* r64);
-__global__ void k_mixed_update(int n, double* x, const float* d, const SolverScalars* sc);
+template <class TV, class TX, class TI, bool FIRST>
+__global__ void k_krylov_1t(KrylovArgs<TV, TX, TI> a, const SolverScalars* sc);
+template <class TV, class TX, class TI>
+__global__ void k_krylov_2(KrylovArgs<TV, TX, TI> a, const SolverScalars* sc);
+template <class TV, class TX, class TI>
+__global__ void k_krylov_3(KrylovArgs<TV, TX, TI> a, const SolverScalars* sc);
+__global__ void k_finish(int kind, int nbp, const double* part, SolverScalars* sc, SolveCtl ctl);
+__global__ void k_resid(ResidArgs a, int mode, double* rs, double* rhat);
+template <class TX>
+__global__ void k_mx_start(int n, const double* r64, TX* rs, TX* rhat, TX* xs, double* part,
+                           const SolverScalars* sc);
+template <class TX>
+__global__ void k_mx_replace(int n, const double* r64, TX* rs, const TX* rhat, double* part,
+                             const SolverScalars* sc);
+template <class TX>
+__global__ void k_mx_flush(int n, double* x64, TX* xs);
+__global__ void k_to_short(int n, size_t ld, const int* nbr, const int* cnt, short* out,
+                           uint32_t* err);
 __global__ void k_comp_vel(int n, KernelConsts kc, size_t ld, StateView s, const int* nbr,
                            const int* cnt, const double* L, const SimScalars* ss, uint32_t* err);
 __global__ void k_comp_pos(int n, StateView s, const SimScalars* ss);
@@ -138,8 +192,8 @@ __global__ void k_freesurf(int n, double thr, size_t ld, StateView s, const int*
 __global__ void k_shift(int n, KernelConsts kc, double csh, double cap, size_t ld, StateView s,
                         const int* nbr, const int* cnt, const double* L);
 __global__ void k_shift_apply(int n, StateView s);
-__global__ void k_maxes(int n, const double* velm, const double* nut, unsigned long long* out);
-__global__ void k_timestep(const unsigned long long* mx, SimScalars* ss, double dl, double s1,
+__global__ void k_maxes(int n, const double* velm, const double* nut, double* part);
+__global__ void k_timestep(int nb, const double* part, SimScalars* ss, double dl, double s1,
                            double mu, double rho, int adaptive);
 __global__ void k_advance_time(SimScalars* ss);
 __global__ void k_copy(const double2* a, double2* b, size_t n);

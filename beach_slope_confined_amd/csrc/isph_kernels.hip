@@ -12,65 +12,6 @@
 
 namespace isph {
 
-// ---------------------------------------------------------------------------
-// reductions in a fixed tree: wave64 butterfly (xor 1,2,..,32), the four
-// waves as (w0+w1)+(w2+w3), block partials finished by the last block
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int s = 1; s < 64; s <<= 1) v = v + __shfl_xor(v, s, 64);
-    return v;
-}
-
-// Logical block of a solver launch: physical blocks are dealt round-robin
-// to the 8 XCDs, so XCD x gets the contiguous logical range
-// [x*per, (x+1)*per) and its L2 holds the vector entries its rows gather.
-__device__ __forceinline__ int xcd_block(int bid, int nblocks) {
-    int per = nblocks >> 3;  // nblocks is a multiple of 8
-    return (bid & 7) * per + (bid >> 3);
-}
-
-template <int NQ>
-__device__ __forceinline__ bool reduce_tree(const double (&v)[NQ], double* part, int nbp, int lb,
-                                            uint32_t* counter, double (&out)[NQ]) {
-    __shared__ double red[NQ][4];
-    __shared__ int amlast;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double s = wave_sum(v[q]);
-        if (lane == 0) red[q][w] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q)
-            __hip_atomic_store(&part[q * nbp + lb], (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        uint32_t t = atomicAdd(counter, 1u);
-        amlast = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!amlast) return false;
-    __threadfence();
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double s = 0.0;
-        for (int k = threadIdx.x; k < nbp; k += BLOCK)
-            s = s + __hip_atomic_load(&part[q * nbp + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s = wave_sum(s);
-        __syncthreads();
-        if (lane == 0) red[q][w] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) out[q] = (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]);
-    if (threadIdx.x == 0) atomicExch(counter, 0u);
-    return true;
-}
-
 __device__ __forceinline__ void raise_err(uint32_t* err, uint32_t bit) { atomicOr(err, bit); }
 
 // ---------------------------------------------------------------------------
@@ -590,253 +531,6 @@ __global__ void __launch_bounds__(BLOCK) k_assemble(int n, KernelConsts kc, doub
 }
 
 // ---------------------------------------------------------------------------
-// BiCGSTAB, recurrences of solver.f90:162-213 (van der Vorst 1992), on the
-// Jacobi-scaled matrix; T = double (FP64 solve) or float (mixed-precision
-// inner solve, dot products accumulated in double).
-// ---------------------------------------------------------------------------
-template <class T, class F>
-__device__ __forceinline__ T spmv_row(int i, int c, int dp, const int* __restrict__ nbr,
-                                      const T* __restrict__ val, size_t ld, T xi, F&& xj) {
-    T s = (T)0;
-    for (int k = 0; k < c; ++k) {
-        if (k == dp) s = s + xi;
-        const int j = nbr[(size_t)k * ld + i];
-        s = s + val[(size_t)k * ld + i] * xj(j);
-    }
-    if (dp == c) s = s + xi;
-    return s;
-}
-
-// init: res = A x0, rs = b - res, rhat = rs; dots (b,b), (rs,rs)
-__global__ void __launch_bounds__(BLOCK) k_bicg_init(SolveArgs a, SolverScalars* sc) {
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int i = lb * BLOCK + threadIdx.x;
-    double vb = 0.0, vr = 0.0;
-    if (i < a.n) {
-        const double* x = a.x;
-        double s = spmv_row<double>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, x[i],
-                                    [&](int j) { return x[j]; });
-        double bi = a.b[i];
-        double r = bi - s;
-        a.rs[i] = r;
-        a.rhat[i] = r;
-        vb = bi * bi;
-        vr = r * r;
-    }
-    double v[2] = {vb, vr}, out[2];
-    if (reduce_tree<2>(v, a.part, gridDim.x, lb, &a.counters[0], out) && threadIdx.x == 0) {
-        sc->bb = out[0];
-        sc->nb = sqrt(out[0]);
-        sc->rr = out[1];
-        sc->prevr = out[1];
-        sc->iters = 0;
-        sc->first = 1;
-        sc->status = 0;
-        if (out[0] == 0.0) {
-            sc->done = DONE_ZERO_RHS;
-            sc->relres = 0.0;
-        } else if (sqrt(out[1]) <= a.tol * sc->nb) {
-            sc->done = DONE_CONVERGED;
-            sc->relres = sqrt(out[1]) / sc->nb;
-        } else {
-            sc->done = 0;
-        }
-    }
-}
-
-// iteration step 1: p = rs + beta (p - w mm1) (or p = rs after a (re)start),
-// mm1 = A p, alpha = (rhat, rs) / (mm1, rhat)
-template <class T>
-__global__ void __launch_bounds__(BLOCK) k_bicg_1(SolveArgsT<T> a, SolverScalars* sc,
-                                                  const T* __restrict__ p_old,
-                                                  const T* __restrict__ mm1_old,
-                                                  T* __restrict__ p_new, T* __restrict__ mm1_new) {
-    if (sc->done) return;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int i = lb * BLOCK + threadIdx.x;
-    const bool first = sc->first != 0;
-    const T beta = (T)sc->betas, w = (T)sc->wi;
-    const T* rs = a.rs;
-    auto pn = [&](int j) -> T {
-        return first ? rs[j] : rs[j] + beta * (p_old[j] - w * mm1_old[j]);
-    };
-    double v = 0.0;
-    if (i < a.n) {
-        T pi = pn(i);
-        T s = spmv_row<T>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, pi, pn);
-        p_new[i] = pi;
-        mm1_new[i] = s;
-        v = (double)s * (double)a.rhat[i];
-    }
-    double vv[1] = {v}, out[1];
-    if (reduce_tree<1>(vv, a.part, gridDim.x, lb, &a.counters[1], out) && threadIdx.x == 0) {
-        sc->alpha = sc->prevr / out[0];
-    }
-}
-
-// iteration step 2: si = rs - alpha mm1, mm2 = A si, w = (mm2,si)/(mm2,mm2)
-template <class T>
-__global__ void __launch_bounds__(BLOCK) k_bicg_2(SolveArgsT<T> a, SolverScalars* sc,
-                                                  const T* __restrict__ mm1) {
-    if (sc->done) return;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int i = lb * BLOCK + threadIdx.x;
-    const T alpha = (T)sc->alpha;
-    const T* rs = a.rs;
-    auto sfn = [&](int j) -> T { return rs[j] - alpha * mm1[j]; };
-    double v0 = 0.0, v1 = 0.0;
-    if (i < a.n) {
-        T si = sfn(i);
-        T s = spmv_row<T>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, si, sfn);
-        a.si[i] = si;
-        a.mm2[i] = s;
-        v0 = (double)s * (double)si;
-        v1 = (double)s * (double)s;
-    }
-    double vv[2] = {v0, v1}, out[2];
-    if (reduce_tree<2>(vv, a.part, gridDim.x, lb, &a.counters[2], out) && threadIdx.x == 0) {
-        sc->wi = out[0] / out[1];
-    }
-}
-
-// iteration step 3: x += alpha p + w si, rs = si - w mm2,
-// beta = ((rs,rhat)/prevr)(alpha/w), stop tests
-template <class T>
-__global__ void __launch_bounds__(BLOCK) k_bicg_3(SolveArgsT<T> a, SolverScalars* sc,
-                                                  const T* __restrict__ p, int inner) {
-    if (sc->done) return;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int i = lb * BLOCK + threadIdx.x;
-    const T alpha = (T)sc->alpha, w = (T)sc->wi;
-    double v0 = 0.0, v1 = 0.0;
-    if (i < a.n) {
-        T si = a.si[i];
-        T x = (a.xs[i] + alpha * p[i]) + w * si;
-        T r = si - w * a.mm2[i];
-        a.xs[i] = x;
-        a.rs[i] = r;
-        v0 = (double)r * (double)a.rhat[i];
-        v1 = (double)r * (double)r;
-    }
-    double vv[2] = {v0, v1}, out[2];
-    if (reduce_tree<2>(vv, a.part, gridDim.x, lb, &a.counters[3], out) && threadIdx.x == 0) {
-        double betas = (out[0] / sc->prevr) * (sc->alpha / sc->wi);
-        sc->betas = betas;
-        sc->prevr = out[0];
-        sc->rr = out[1];
-        sc->iters = sc->iters + 1;
-        sc->total = sc->total + 1;
-        sc->first = 0;
-        if (!isfinite(betas) || !isfinite(out[1])) {
-            sc->done = inner ? DONE_INNER : DONE_FAILED;
-            if (!inner) sc->status = ERR_NOCONV;
-        } else if (!inner && sqrt(out[1]) <= a.tol * sc->nb) {
-            sc->done = DONE_RECURRENCE;
-        } else if (inner && sqrt(out[1]) <= a.tol * sqrt(sc->rr0)) {
-            sc->done = DONE_INNER;
-        } else if (sc->total >= a.maxit || (inner && sc->iters >= a.inner_maxit)) {
-            sc->done = inner ? DONE_INNER : DONE_FAILED;
-            if (!inner) sc->status = ERR_NOCONV;
-        }
-    }
-}
-
-// true residual r = b - A x after the recurrence residual passed; restart
-// from it when it did not
-__global__ void __launch_bounds__(BLOCK) k_bicg_true(SolveArgs a, SolverScalars* sc) {
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int i = lb * BLOCK + threadIdx.x;
-    double v = 0.0;
-    if (i < a.n) {
-        const double* x = a.x;
-        double s = spmv_row<double>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, x[i],
-                                    [&](int j) { return x[j]; });
-        double r = a.b[i] - s;
-        a.rs[i] = r;
-        a.rhat[i] = r;
-        v = r * r;
-    }
-    double vv[1] = {v}, out[1];
-    if (reduce_tree<1>(vv, a.part, gridDim.x, lb, &a.counters[4], out) && threadIdx.x == 0) {
-        double tr = out[0];
-        if (sqrt(tr) <= a.tol * sc->nb) {
-            sc->done = DONE_CONVERGED;
-            sc->relres = sqrt(tr) / sc->nb;
-        } else {
-            sc->rr = tr;
-            sc->prevr = tr;
-            sc->first = 1;
-            sc->done = 0;
-            if (sc->total >= a.maxit) {
-                sc->done = DONE_FAILED;
-                sc->status = ERR_NOCONV;
-                sc->relres = sqrt(tr) / sc->nb;
-            }
-        }
-    }
-}
-
-// mixed precision, outer step: r64 = b - A64 x64, ||r64||; converged?
-__global__ void __launch_bounds__(BLOCK) k_mixed_resid(SolveArgs a, SolverScalars* sc,
-                                                       double* __restrict__ r64) {
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int i = lb * BLOCK + threadIdx.x;
-    double vb = 0.0, vr = 0.0;
-    if (i < a.n) {
-        const double* x = a.x;
-        double s = spmv_row<double>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, x[i],
-                                    [&](int j) { return x[j]; });
-        double bi = a.b[i];
-        double r = bi - s;
-        r64[i] = r;
-        vb = bi * bi;
-        vr = r * r;
-    }
-    double vv[2] = {vb, vr}, out[2];
-    if (reduce_tree<2>(vv, a.part, gridDim.x, lb, &a.counters[5], out) && threadIdx.x == 0) {
-        sc->bb = out[0];
-        sc->nb = sqrt(out[0]);
-        sc->nr = sqrt(out[1]);
-        sc->relres = sc->bb == 0.0 ? 0.0 : sc->nr / sc->nb;
-        if (out[0] == 0.0) sc->done = DONE_ZERO_RHS;
-        else if (sc->nr <= a.tol * sc->nb) sc->done = DONE_CONVERGED;
-        else sc->done = 0;
-    }
-}
-
-// mixed precision, inner start: rs = float(r64/||r64||), rhat = rs, x32 = 0
-__global__ void __launch_bounds__(BLOCK) k_mixed_start(SolveArgsT<float> f, SolverScalars* sc,
-                                                       const double* __restrict__ r64) {
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int i = lb * BLOCK + threadIdx.x;
-    double v = 0.0;
-    if (i < f.n) {
-        const double inv = 1.0 / sc->nr;
-        float r = (float)(r64[i] * inv);
-        f.rs[i] = r;
-        f.rhat[i] = r;
-        f.xs[i] = 0.0f;
-        v = (double)r * (double)r;
-    }
-    double vv[1] = {v}, out[1];
-    if (reduce_tree<1>(vv, f.part, gridDim.x, lb, &f.counters[6], out) && threadIdx.x == 0) {
-        sc->prevr = out[0];
-        sc->rr0 = out[0];
-        sc->first = 1;
-        sc->iters = 0;
-        sc->done = 0;
-    }
-}
-
-// mixed precision, refinement: x64 += double(x32) * ||r64||
-__global__ void k_mixed_update(int n, double* __restrict__ x, const float* __restrict__ d,
-                               const SolverScalars* sc) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    x[i] = x[i] + (double)d[i] * sc->nr;
-}
-
-// ---------------------------------------------------------------------------
 // comp_vel, integrator.f90:715-865
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(BLOCK) k_comp_vel(int n, KernelConsts kc, size_t ld,
@@ -986,7 +680,7 @@ __global__ void k_shift_apply(int n, StateView s) {
 // (max is order-independent; non-negative doubles compare as uint64)
 // ---------------------------------------------------------------------------
 __global__ void k_maxes(int n, const double* __restrict__ velm, const double* __restrict__ nut,
-                        unsigned long long* out) {
+                        double* __restrict__ part) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     double a = 0.0, b = 0.0;
     if (i < n) {
@@ -995,21 +689,26 @@ __global__ void k_maxes(int n, const double* __restrict__ velm, const double* __
         if (!(a == a)) a = 0.0;
         if (!(b == b)) b = 0.0;
     }
-    for (int s = 1; s < 64; s <<= 1) {
-        a = fmax(a, __shfl_xor(a, s, 64));
-        b = fmax(b, __shfl_xor(b, s, 64));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(&out[0], (unsigned long long)__double_as_longlong(a));
-        atomicMax(&out[1], (unsigned long long)__double_as_longlong(b));
+    a = block_max(a);
+    b = block_max(b);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[gridDim.x + blockIdx.x] = b;
     }
 }
 
-__global__ void k_timestep(const unsigned long long* mx, SimScalars* ss, double dl, double s1,
-                           double mu, double rho, int adaptive) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const double umax = __longlong_as_double((long long)mx[0]);
-    const double numax = __longlong_as_double((long long)mx[1]);
+__global__ void k_timestep(int nb, const double* __restrict__ part, SimScalars* ss, double dl,
+                           double s1, double mu, double rho, int adaptive) {
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nb; k += BLOCK) {
+        a = fmax(a, part[k]);
+        b = fmax(b, part[nb + k]);
+    }
+    a = block_max(a);
+    b = block_max(b);
+    if (threadIdx.x != 0) return;
+    const double umax = a;
+    const double numax = b;
     ss->umax = umax;
     ss->numax = numax;
     if (adaptive) {
@@ -1044,15 +743,5 @@ __global__ void k_copy(const double2* __restrict__ a, double2* __restrict__ b, s
     size_t stride = (size_t)gridDim.x * BLOCK;
     for (; i < n; i += stride) b[i] = a[i];
 }
-
-// explicit instantiations
-template __global__ void k_bicg_1<double>(SolveArgsT<double>, SolverScalars*, const double*,
-                                          const double*, double*, double*);
-template __global__ void k_bicg_1<float>(SolveArgsT<float>, SolverScalars*, const float*,
-                                         const float*, float*, float*);
-template __global__ void k_bicg_2<double>(SolveArgsT<double>, SolverScalars*, const double*);
-template __global__ void k_bicg_2<float>(SolveArgsT<float>, SolverScalars*, const float*);
-template __global__ void k_bicg_3<double>(SolveArgsT<double>, SolverScalars*, const double*, int);
-template __global__ void k_bicg_3<float>(SolveArgsT<float>, SolverScalars*, const float*, int);
 
 }  // namespace isph

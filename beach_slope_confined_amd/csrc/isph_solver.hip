@@ -1,0 +1,393 @@
+// isph_solver.hip -- BiCGSTAB for the Jacobi-scaled PPE on gfx950.
+//
+// Recurrences of the reference's bicgstab (solver.f90:162-213, van der
+// Vorst 1992).  One iteration is three row kernels, each followed by a
+// one-block reduction kernel (k_finish) that turns the block partials into
+// the iteration's scalars and stop flags:
+//   k_krylov_1  p = r + beta q (or p = r after a (re)start), v = A p, (v, rhat)
+//   k_krylov_2  s = r - alpha v, t = A s, (t, s), (t, t)
+//   k_krylov_3  x += alpha p + omega s, r = s - omega t, q = p - omega v,
+//               (r, rhat), (r, r)
+// q carries p - omega v from one iteration to the next so that the p update
+// costs two gathers (r_j, q_j) instead of three, with the same rounding as
+// r + beta (p - omega v).  Partials are plain stores; the kernel boundary
+// makes them visible, so no fences or global atomics are needed.
+//
+// Flavours (template TV matrix value, TX vector, TI column index):
+//   <double, double, int>    FP64 drop-in
+//   <float,  float,  short>  mixed: FP32 values and FP32 Krylov vectors
+//   <float,  double, short>  mixed: FP32 values, FP64 Krylov vectors
+// short indices are offsets j - i (built only when every offset fits).  The
+// mixed flavours add reliable updates: when the recurrence residual has
+// dropped by `delta`, x is flushed into the FP64 solution and the FP64 true
+// residual b - A64 x replaces the recurrence residual (search direction
+// kept).  Dot products always accumulate in FP64.
+#include "isph_kernels.h"
+
+namespace isph {
+
+__device__ __forceinline__ int col_of(int e, int) { return e; }
+__device__ __forceinline__ int col_of(short e, int i) { return i + (int)e; }
+
+// Row i of A x with the unit diagonal at position dp, summed in column order
+// (spmv2, solver.f90:84-127).  Four entries are loaded ahead of their adds.
+template <class TV, class TX, class TI, class F>
+__device__ __forceinline__ TX spmv_row(int i, int c, int dp, const TI* __restrict__ nbr,
+                                       const TV* __restrict__ val, size_t ld, TX xi, F&& xj) {
+    TX s = (TX)0;
+    int k = 0;
+    for (; k + 4 <= c; k += 4) {
+        const size_t o = (size_t)k * ld + i;
+        const int j0 = col_of(nbr[o], i), j1 = col_of(nbr[o + ld], i);
+        const int j2 = col_of(nbr[o + 2 * ld], i), j3 = col_of(nbr[o + 3 * ld], i);
+        const TX v0 = (TX)val[o], v1 = (TX)val[o + ld], v2 = (TX)val[o + 2 * ld],
+                 v3 = (TX)val[o + 3 * ld];
+        const TX x0 = xj(j0), x1 = xj(j1), x2 = xj(j2), x3 = xj(j3);
+        if (k == dp) s = s + xi;
+        s = s + v0 * x0;
+        if (k + 1 == dp) s = s + xi;
+        s = s + v1 * x1;
+        if (k + 2 == dp) s = s + xi;
+        s = s + v2 * x2;
+        if (k + 3 == dp) s = s + xi;
+        s = s + v3 * x3;
+    }
+    for (; k < c; ++k) {
+        const size_t o = (size_t)k * ld + i;
+        const int j = col_of(nbr[o], i);
+        const TX v = (TX)val[o];
+        const TX x = xj(j);
+        if (k == dp) s = s + xi;
+        s = s + v * x;
+    }
+    if (dp == c) s = s + xi;
+    return s;
+}
+
+template <class TV, class TX, class TI, bool FIRST>
+__global__ void __launch_bounds__(BLOCK) k_krylov_1t(KrylovArgs<TV, TX, TI> a,
+                                                     const SolverScalars* sc) {
+    if (sc->done) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    const TX beta = (TX)sc->betas;
+    const TX* __restrict__ rs = a.rs;
+    const TX* __restrict__ q = a.q;
+    auto pn = [&](int j) -> TX { return FIRST ? rs[j] : rs[j] + beta * q[j]; };
+    double v = 0.0;
+    if (i < a.n) {
+        const TX pi = pn(i);
+        const TX s = spmv_row<TV, TX, TI>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, pi, pn);
+        a.p[i] = pi;
+        a.mm1[i] = s;
+        v = (double)s * (double)a.rhat[i];
+    }
+    v = block_sum(v);
+    if (threadIdx.x == 0) a.part[lb] = v;
+}
+
+template <class TV, class TX, class TI>
+__global__ void __launch_bounds__(BLOCK) k_krylov_2(KrylovArgs<TV, TX, TI> a,
+                                                    const SolverScalars* sc) {
+    if (sc->done) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    const TX alpha = (TX)sc->alpha;
+    const TX* __restrict__ rs = a.rs;
+    const TX* __restrict__ mm1 = a.mm1;
+    auto sf = [&](int j) -> TX { return rs[j] - alpha * mm1[j]; };
+    double v0 = 0.0, v1 = 0.0;
+    if (i < a.n) {
+        const TX si = sf(i);
+        const TX s = spmv_row<TV, TX, TI>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, si, sf);
+        a.si[i] = si;
+        a.mm2[i] = s;
+        v0 = (double)s * (double)si;
+        v1 = (double)s * (double)s;
+    }
+    v0 = block_sum(v0);
+    v1 = block_sum(v1);
+    if (threadIdx.x == 0) {
+        a.part[lb] = v0;
+        a.part[gridDim.x + lb] = v1;
+    }
+}
+
+template <class TV, class TX, class TI>
+__global__ void __launch_bounds__(BLOCK) k_krylov_3(KrylovArgs<TV, TX, TI> a,
+                                                    const SolverScalars* sc) {
+    if (sc->done) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    const TX alpha = (TX)sc->alpha, w = (TX)sc->wi;
+    double v0 = 0.0, v1 = 0.0;
+    if (i < a.n) {
+        const TX si = a.si[i], p = a.p[i];
+        const TX x = (a.xs[i] + alpha * p) + w * si;
+        const TX r = si - w * a.mm2[i];
+        const TX qq = p - w * a.mm1[i];
+        a.xs[i] = x;
+        a.rs[i] = r;
+        a.q[i] = qq;
+        v0 = (double)r * (double)a.rhat[i];
+        v1 = (double)r * (double)r;
+    }
+    v0 = block_sum(v0);
+    v1 = block_sum(v1);
+    if (threadIdx.x == 0) {
+        a.part[lb] = v0;
+        a.part[gridDim.x + lb] = v1;
+    }
+}
+
+// One block: finish the reductions of the preceding row kernel and update
+// the solver scalars.  Thread t sums partials t, t+256, ... in order, then
+// the block tree.
+__global__ void __launch_bounds__(BLOCK) k_finish(int kind, int nbp, const double* __restrict__ part,
+                                                  SolverScalars* sc, SolveCtl ctl) {
+    if ((kind == FIN_ALPHA || kind == FIN_OMEGA || kind == FIN_BETA) && sc->done) return;
+    const bool two = kind != FIN_ALPHA && kind != FIN_START && kind != FIN_REPLACE;
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = threadIdx.x; k < nbp; k += BLOCK) {
+        s0 = s0 + part[k];
+        if (two) s1 = s1 + part[nbp + k];
+    }
+    const double t0 = block_sum(s0);
+    const double t1 = two ? block_sum(s1) : 0.0;
+    if (threadIdx.x != 0) return;
+    switch (kind) {
+        case FIN_INIT: {
+            sc->bb = t0;
+            sc->nb = sqrt(t0);
+            sc->rr = t1;
+            sc->prevr = t1;
+            sc->iters = 0;
+            sc->total = 0;
+            sc->first = 1;
+            sc->status = 0;
+            sc->nupd = 0;
+            if (t0 == 0.0) {
+                sc->done = DONE_ZERO_RHS;
+                sc->relres = 0.0;
+            } else if (sqrt(t1) <= ctl.tol * sc->nb) {
+                sc->done = DONE_CONVERGED;
+                sc->relres = sqrt(t1) / sc->nb;
+            } else {
+                sc->done = 0;
+            }
+            break;
+        }
+        case FIN_ALPHA:
+            sc->alpha = sc->prevr / t0;
+            break;
+        case FIN_OMEGA:
+            sc->wi = t0 / t1;
+            break;
+        case FIN_BETA: {
+            const double betas = (t0 / sc->prevr) * (sc->alpha / sc->wi);
+            sc->betas = betas;
+            sc->prevr_prev = sc->prevr;
+            sc->prevr = t0;
+            sc->rr = t1;
+            sc->iters = sc->iters + 1;
+            sc->total = sc->total + 1;
+            sc->first = 0;
+            const double nr = sqrt(t1);
+            if (!isfinite(betas) || !isfinite(t1)) {
+                sc->done = DONE_FAILED;
+                sc->status = ERR_NOCONV;
+                sc->relres = nr / sc->nb;
+            } else if (!ctl.mixed && nr <= ctl.tol * sc->nb) {
+                sc->done = DONE_RECURRENCE;
+            } else if (ctl.mixed && (nr <= ctl.tol * sc->nb || nr <= ctl.delta * sc->rupd)) {
+                sc->done = DONE_UPDATE;
+            } else if (sc->total >= ctl.maxit) {
+                sc->done = DONE_FAILED;
+                sc->status = ERR_NOCONV;
+                sc->relres = nr / sc->nb;
+            }
+            break;
+        }
+        case FIN_TRUE: {
+            const double nr = sqrt(t1);
+            if (nr <= ctl.tol * sc->nb) {
+                sc->done = DONE_CONVERGED;
+                sc->relres = nr / sc->nb;
+            } else {
+                sc->rr = t1;
+                sc->prevr = t1;
+                sc->first = 1;
+                sc->done = 0;
+                if (sc->total >= ctl.maxit) {
+                    sc->done = DONE_FAILED;
+                    sc->status = ERR_NOCONV;
+                    sc->relres = nr / sc->nb;
+                }
+            }
+            break;
+        }
+        case FIN_RESID: {
+            // mixed outer residual; nupd counts the flushes (ctl.mixed == 2 after a flush)
+            sc->bb = t0;
+            sc->nb = sqrt(t0);
+            sc->nr = sqrt(t1);
+            if (ctl.mixed == 2) sc->nupd = sc->nupd + 1;
+            if (t0 == 0.0) {
+                sc->done = DONE_ZERO_RHS;
+                sc->relres = 0.0;
+                break;
+            }
+            sc->relres = sc->nr / sc->nb;
+            if (sc->nr <= ctl.tol * sc->nb) {
+                sc->done = DONE_CONVERGED;
+            } else if (ctl.mixed == 2 && (sc->nupd >= ctl.maxupd || sc->total >= ctl.maxit)) {
+                sc->done = DONE_FAILED;
+                sc->status = ERR_NOCONV;
+            } else {
+                sc->done = 0;
+            }
+            break;
+        }
+        case FIN_START:
+            sc->prevr = t0;
+            sc->prevr_prev = t0;
+            sc->rupd = sc->nr;
+            sc->first = 1;
+            sc->iters = 0;
+            sc->done = 0;
+            break;
+        case FIN_REPLACE: {
+            sc->betas = (t0 / sc->prevr_prev) * (sc->alpha / sc->wi);
+            sc->prevr = t0;
+            sc->rupd = sc->nr;
+            sc->done = 0;
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+// r = b - A64 x in FP64; partials (b,b), (r,r).  mode 1 also restarts the
+// FP64 recurrence from r (rs = rhat = r).
+__global__ void __launch_bounds__(BLOCK) k_resid(ResidArgs a, int mode, double* __restrict__ rs,
+                                                 double* __restrict__ rhat) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    double vb = 0.0, vr = 0.0;
+    if (i < a.n) {
+        const double* __restrict__ x = a.x;
+        const double s = spmv_row<double, double, int>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld,
+                                                       x[i], [&](int j) { return x[j]; });
+        const double bi = a.b[i];
+        const double r = bi - s;
+        a.r[i] = r;
+        if (mode == 1) {
+            rs[i] = r;
+            rhat[i] = r;
+        }
+        vb = bi * bi;
+        vr = r * r;
+    }
+    vb = block_sum(vb);
+    vr = block_sum(vr);
+    if (threadIdx.x == 0) {
+        a.part[lb] = vb;
+        a.part[gridDim.x + lb] = vr;
+    }
+}
+
+template <class TX>
+__global__ void __launch_bounds__(BLOCK) k_mx_start(int n, const double* __restrict__ r64,
+                                                    TX* __restrict__ rs, TX* __restrict__ rhat,
+                                                    TX* __restrict__ xs, double* __restrict__ part,
+                                                    const SolverScalars* sc) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const TX r = (TX)r64[i];
+        rs[i] = r;
+        rhat[i] = r;
+        xs[i] = (TX)0;
+        v = (double)r * (double)r;
+    }
+    v = block_sum(v);
+    if (threadIdx.x == 0) part[lb] = v;
+}
+
+template <class TX>
+__global__ void __launch_bounds__(BLOCK) k_mx_replace(int n, const double* __restrict__ r64,
+                                                      TX* __restrict__ rs,
+                                                      const TX* __restrict__ rhat,
+                                                      double* __restrict__ part,
+                                                      const SolverScalars* sc) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const TX r = (TX)r64[i];
+        rs[i] = r;
+        v = (double)r * (double)rhat[i];
+    }
+    v = block_sum(v);
+    if (threadIdx.x == 0) part[lb] = v;
+}
+
+template <class TX>
+__global__ void k_mx_flush(int n, double* __restrict__ x64, TX* __restrict__ xs) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    x64[i] = x64[i] + (double)xs[i];
+    xs[i] = (TX)0;
+}
+
+// 16-bit column offsets j - i for the mixed-precision SpMV
+__global__ void k_to_short(int n, size_t ld, const int* __restrict__ nbr,
+                           const int* __restrict__ cnt, short* __restrict__ out, uint32_t* err) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int c = cnt[i];
+    bool bad = false;
+    for (int k = 0; k < c; ++k) {
+        const int d = nbr[(size_t)k * ld + i] - i;
+        if (d < -32768 || d > 32767) bad = true;
+        out[(size_t)k * ld + i] = (short)d;
+    }
+    if (bad) atomicOr(err, ERR_INDEX16);
+}
+
+// instantiations
+template __global__ void k_krylov_1t<double, double, int, true>(KrylovArgs<double, double, int>,
+                                                               const SolverScalars*);
+template __global__ void k_krylov_1t<double, double, int, false>(KrylovArgs<double, double, int>,
+                                                                const SolverScalars*);
+template __global__ void k_krylov_2<double, double, int>(KrylovArgs<double, double, int>,
+                                                         const SolverScalars*);
+template __global__ void k_krylov_3<double, double, int>(KrylovArgs<double, double, int>,
+                                                         const SolverScalars*);
+#define ISPH_MIXED_FLAVOUR(TX, TI)                                                              \
+    template __global__ void k_krylov_1t<float, TX, TI, true>(KrylovArgs<float, TX, TI>,        \
+                                                              const SolverScalars*);           \
+    template __global__ void k_krylov_1t<float, TX, TI, false>(KrylovArgs<float, TX, TI>,       \
+                                                               const SolverScalars*);          \
+    template __global__ void k_krylov_2<float, TX, TI>(KrylovArgs<float, TX, TI>,               \
+                                                       const SolverScalars*);                  \
+    template __global__ void k_krylov_3<float, TX, TI>(KrylovArgs<float, TX, TI>,               \
+                                                       const SolverScalars*);
+ISPH_MIXED_FLAVOUR(float, short)
+ISPH_MIXED_FLAVOUR(float, int)
+ISPH_MIXED_FLAVOUR(double, short)
+ISPH_MIXED_FLAVOUR(double, int)
+template __global__ void k_mx_start<float>(int, const double*, float*, float*, float*, double*,
+                                           const SolverScalars*);
+template __global__ void k_mx_start<double>(int, const double*, double*, double*, double*, double*,
+                                            const SolverScalars*);
+template __global__ void k_mx_replace<float>(int, const double*, float*, const float*, double*,
+                                             const SolverScalars*);
+template __global__ void k_mx_replace<double>(int, const double*, double*, const double*, double*,
+                                              const SolverScalars*);
+template __global__ void k_mx_flush<float>(int, double*, float*);
+template __global__ void k_mx_flush<double>(int, double*, double*);
+
+}  // namespace isph

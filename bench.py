@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""ISPH step throughput on one MI355X.
+
+Workload (the single-step-at-scale configuration): 1,048,576 fluid
+particles on a jittered 1024x1024 lattice in the unit square with three
+wall layers on the left, right and bottom, 16-mode Fourier velocity field,
+Wendland C2 kernel, h = 1.3 dx, BiCGSTAB to a true relative residual of
+1e-8.  A step is the full ISPH step of the reference (main.f90:41-66):
+projection, re-binning, neighbour search + KGC, porous drag, eddy
+viscosity, v*, PPE assembly, PPE solve, pressure-gradient correction,
+position update, re-binning, neighbour search, free surface, porous drag,
+shifting, timestep reduction.
+
+Prints one JSON line.  `value` is fluid particle-steps/s of the FP64 step;
+the mixed-precision PPE step on the same input is reported beside it
+(`mixed`), with its pressure and velocity error against the FP64 step.
+`roofline` is the dominant kernel (a BiCGSTAB SpMV kernel) timed with HIP
+events on the library's stream during the timed steps; `cpu_baseline` is
+the CPU oracle (this repository's C restatement of the reference step,
+OpenMP) on the host cores of the same machine.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ISPH particle-steps/s @1M (FP64 vs mixed-prec PPE); adjoint/forward cost"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nside", type=int, default=1024)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=400,
+                    help="BiCGSTAB iterations timed on the CPU oracle")
+    return ap.parse_args()
+
+
+def kernel_bytes(kind, nnz_off, n, val_bytes, vec_bytes):
+    """Algorithmic bytes of one launch of a BiCGSTAB SpMV kernel: every
+    neighbour index and matrix value once, per row cnt and dpos, each
+    gathered vector counted once per row, own-row reads and writes once."""
+    idx = 4 * nnz_off + 8 * n            # nbr + cnt + dpos
+    val = val_bytes * nnz_off
+    if kind == "p_spmv":                 # gathers r, q; reads rhat; writes p, v
+        vec = vec_bytes * (2 + 1 + 2) * n
+    else:                                # gathers r, v; writes s, t
+        vec = vec_bytes * (2 + 2) * n
+    return idx + val + vec
+
+
+def run_gpu(cfg, steps, warmup, precision, profile):
+    from beach_slope_confined_amd import capi, configs
+    from beach_slope_confined_amd.params import PREC_MIXED
+    c = dict(cfg)
+    c["params"] = cfg["params"].copy(precision=precision)
+    sim = configs.make_sim(c, capi.IsphSim)
+    # every step starts from the same device-resident input (the workload is
+    # one ISPH step at scale; the snapshot copy is device-to-device)
+    sim.checkpoint(0)
+    iters = []
+    first = None
+    for _ in range(warmup):
+        sim.restore(0)
+        sim.step(1)
+        iters.append(sim.scalars()["iters"])
+        if first is None:
+            first = sim.get_particles()
+    if profile:
+        sim.profile_enable(True)
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.restore(0)
+        sim.step(1)
+        iters.append(sim.scalars()["iters"])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    s = sim.scalars()
+    ptr, _ = sim.get_neighbours()
+    out = dict(sim=sim, seconds=t1 - t0, iters=iters, relres=s["relres"], outer=s["outer"],
+               nnz_off=int(ptr[-1]), n=len(ptr) - 1, first=first)
+    if profile:
+        k0, k1 = (2, 3) if precision == PREC_MIXED else (0, 1)
+        out["ktime"] = {"p_spmv": sim.kernel_time(k0), "s_spmv": sim.kernel_time(k1)}
+    return out
+
+
+def cpu_baseline(cfg, gpu_iters):
+    """Oracle step on the host cores: every stage once at full size plus
+    `cpu_iters` BiCGSTAB iterations; the solve is scaled to the GPU's
+    iteration count (the oracle's iterates are bit-identical to the GPU's)."""
+    from beach_slope_confined_amd import configs
+    from oracle import pyoracle
+    from oracle.pyoracle import OracleSim
+    threads = pyoracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    sim = configs.make_sim(cfg, OracleSim)
+    t = {}
+
+    def timed(name, fn):
+        a = time.perf_counter()
+        fn()
+        t[name] = t.get(name, 0.0) + time.perf_counter() - a
+
+    s = sim.scalars()
+    sim.set_scalars(s["umax"], s["dt"], s["t"] + s["dt"])
+    for name in ("projection", "cellshift", "resetid", "neighbour", "effpor", "eddyvis",
+                 "int_vel", "ppe_assemble"):
+        timed(name, getattr(sim, name))
+    prm = sim.params.copy(maxit=args.cpu_iters)
+    sim.set_params(prm)
+    a = time.perf_counter()
+    try:
+        sim.ppe_solve()
+    except Exception:
+        pass  # stops at maxit by design
+    t_iter = (time.perf_counter() - a) / args.cpu_iters
+    for name in ("comp_vel", "comp_pos", "cellshiftalt", "neighbour", "freesurf", "effpor",
+                 "opt2_shift", "timestep"):
+        timed(name + "_2" if name in t else name, getattr(sim, name))
+    fixed = sum(t.values())
+    step = fixed + gpu_iters * t_iter
+    sample = (f"oracle stages once at {len(cfg['parts']['x'])} particles ({fixed:.2f} s) + "
+              f"{args.cpu_iters} BiCGSTAB iterations ({t_iter * 1e3:.1f} ms each), solve scaled "
+              f"to the GPU's {gpu_iters} iterations")
+    return dict(value=cfg["n_fluid"] / step, unit="particle-steps/s", cores=threads,
+                kind="port", sample=sample, seconds_per_step=step)
+
+
+def main():
+    global args
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(the multi-GPU decomposition is described in DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+    torch.cuda.init()
+    from beach_slope_confined_amd import configs
+    from beach_slope_confined_amd.params import PREC_FP64, PREC_MIXED, PREC_MIXED_V64
+
+    cfg = configs.tank_1m(args.nside)
+    n_fluid = cfg["n_fluid"]
+    n_total = len(cfg["parts"]["x"])
+
+    r64 = run_gpu(cfg, args.steps, args.warmup, PREC_FP64, True)
+    ms = 1e3 * r64["seconds"] / args.steps
+    value = n_fluid * args.steps / r64["seconds"]
+    timed_iters = r64["iters"][args.warmup:]
+
+    # roofline of the dominant kernel (larger total time of the two SpMV kernels)
+    kt = r64["ktime"]
+    dom = max(kt, key=lambda k: kt[k][1])
+    launches, tot = kt[dom]
+    avg_s = tot / max(launches, 1) / 1e3
+    nbytes = kernel_bytes(dom, r64["nnz_off"], r64["n"], 8, 8)
+    achieved = nbytes / avg_s / 1e9 if launches else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"k_krylov_{1 if dom == 'p_spmv' else 2}<double,double,int>",
+                "launches": launches, "avg_us": round(avg_s * 1e6, 2),
+                "bytes_per_launch": nbytes}
+    pmc = os.path.join(ROOT, "profiles", "bicg_spmv_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            d = json.load(f)
+        roofline["traffic"] = d.get(roofline["kernel"], {}).get("hbm_bytes_per_launch")
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"tank_{args.nside}x{args.nside}: full ISPH step, FP64 PPE "
+                               f"(BiCGSTAB, true residual 1e-8)",
+                   "n_fluid": n_fluid, "n_total": n_total, "global_batch": n_fluid,
+                   "seq_len": 1, "parallelism": "single GPU",
+                   "kernel": "wendland_c2", "h_over_dx": 1.3, "tol": 1e-8},
+        "ppe_iters_per_step": timed_iters, "relres": r64["relres"],
+        "particle_steps_per_s_total": round(n_total * args.steps / r64["seconds"], 1),
+        "roofline": roofline,
+    }
+
+    if not args.no_mixed:
+        for key, prec, vb in (("mixed", PREC_MIXED_V64, 8), ("mixed_f32vec", PREC_MIXED, 4)):
+            try:
+                rmx = run_gpu(cfg, args.steps, args.warmup, prec, True)
+            except Exception as e:  # reported, not hidden
+                line[key] = {"error": str(e)}
+                continue
+            vmx = n_fluid * args.steps / rmx["seconds"]
+            a, b = r64["first"], rmx["first"]
+            f = a["tid"] == 3
+            perr = float(np.abs(b["pressure"] - a["pressure"]).max() / np.abs(a["pressure"]).max())
+            vmag = np.hypot(a["vx"][f], a["vy"][f]).max()
+            verr = float(max(np.abs(b["vx"] - a["vx"])[f].max(),
+                             np.abs(b["vy"] - a["vy"])[f].max()) / vmag)
+            ktm = rmx["ktime"]
+            domm = max(ktm, key=lambda k: ktm[k][1])
+            lm, tm = ktm[domm]
+            avgm = tm / max(lm, 1) / 1e3
+            # FP32 values, 16-bit column offsets, vectors of vb bytes
+            bm = kernel_bytes(domm, rmx["nnz_off"], rmx["n"], 4, vb) - 2 * rmx["nnz_off"]
+            line[key] = {
+                "value": round(vmx, 1), "ms_per_step": round(1e3 * rmx["seconds"] / args.steps, 3),
+                "speedup_vs_fp64": round(vmx / value, 3),
+                "krylov_vectors": "fp64" if vb == 8 else "fp32",
+                "iters_per_step": rmx["iters"][args.warmup:], "reliable_updates": rmx["outer"],
+                "relres": rmx["relres"], "pressure_err_vs_fp64": perr,
+                "velocity_err_vs_fp64": verr,
+                "dominant_kernel": {"kernel": f"k_krylov_{1 if domm == 'p_spmv' else 2}<float,"
+                                              f"{'double' if vb == 8 else 'float'},short>",
+                                    "avg_us": round(avgm * 1e6, 2),
+                                    "achieved_GBs": round(bm / avgm / 1e9, 1) if lm else None},
+            }
+            rmx["sim"].close()
+
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(cfg, int(np.mean(timed_iters)))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

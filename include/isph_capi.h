@@ -56,8 +56,11 @@ extern "C" {
 #define ISPH_DRAG_REFERENCE 0  /* porous.f90:192,204-212: lamk = alpha, vis(oden) */
 #define ISPH_DRAG_ERGUN     1  /* R = -alpha nu (1-n)^2/(n^3 d50^2) - beta |v| (1-n)/(n^3 d50) */
 
-#define ISPH_PREC_FP64  0      /* FP64 matrix, FP64 BiCGSTAB                   */
-#define ISPH_PREC_MIXED 1      /* FP32 matrix + FP32 inner BiCGSTAB, FP64 refinement */
+#define ISPH_PREC_FP64      0  /* FP64 matrix, FP64 BiCGSTAB                        */
+#define ISPH_PREC_MIXED     1  /* FP32 matrix values and FP32 Krylov vectors, FP64
+                                  dot products, FP64 residual + reliable updates     */
+#define ISPH_PREC_MIXED_V64 2  /* FP32 matrix values, FP64 Krylov vectors, FP64
+                                  residual + reliable updates                        */
 
 /* particle flag bits (particle.f90:123-124) */
 #define ISPH_FLAG_DEAD      1
@@ -91,7 +94,8 @@ typedef struct isph_params {
     double beta;       /* Forchheimer coefficient (ISPH_DRAG_ERGUN only)       */
     double d50;        /* grain diameter (porous.f90:111)                      */
     double tol;        /* PPE relative residual tolerance ||b-Ax||/||b||        */
-    double inner_tol;  /* mixed precision: inner FP32 relative tolerance       */
+    double inner_tol;  /* mixed precision: reliable-update factor delta (the FP64
+                          residual replaces the FP32 one each time it fell by delta) */
     double xlcutoff;   /* outflow sink (integrator.f90:60-61), active if sink  */
     double ytcutoff;
     double sig1;       /* adaptive dt factor (initialize.f90:35)               */
@@ -99,9 +103,9 @@ typedef struct isph_params {
     int32_t kernel;      /* ISPH_KERNEL_*                                       */
     int32_t drag_law;    /* ISPH_DRAG_*                                         */
     int32_t precision;   /* ISPH_PREC_*                                         */
-    int32_t maxit;       /* BiCGSTAB iteration cap (FP64, or total inner)       */
-    int32_t inner_maxit; /* mixed precision: inner iterations per refinement    */
-    int32_t max_outer;   /* mixed precision: refinement steps                   */
+    int32_t maxit;       /* BiCGSTAB iteration cap                              */
+    int32_t inner_maxit; /* unused, 0                                           */
+    int32_t max_outer;   /* mixed precision: cap on reliable updates            */
     int32_t max_nbr;     /* neighbour-list capacity per particle                */
     int32_t shifting;    /* 1: opt2_shift moves particles                       */
     int32_t adaptive_dt; /* 1: isph_timestep() updates dt (integrator.f90:950-960) */
@@ -196,6 +200,16 @@ int isph_timestep(isph_ctx* ctx);      /* integrator.f90:908-976             */
 /* nsteps whole steps in the main.f90:41-66 order (no host round trip
  * between stages). */
 int isph_step(isph_ctx* ctx, int32_t nsteps);
+
+/* ---- in-memory checkpoints (no reference counterpart: the reference has no
+ * restart, output.f90 only writes text snapshots) ---------------------------- */
+/* Saves the particle state, the cell order and the scalars (umax, dt, t) to
+ * device-memory slot `slot` (0 <= slot < 64); the slot's memory is
+ * allocated on first use and kept until isph_destroy.  isph_restore puts
+ * the state back; neighbour lists and the PPE matrix are rebuilt by the
+ * next step. */
+int isph_checkpoint(isph_ctx* ctx, int32_t slot);
+int isph_restore(isph_ctx* ctx, int32_t slot);
 
 /* ---- instrumentation (no reference counterpart) -------------------------- */
 /* When enabled, HIP events bracket every launch of the BiCGSTAB SpMV kernels

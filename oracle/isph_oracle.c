@@ -218,8 +218,12 @@ static double tree_finish(const double* part, int64_t nb) {
     return block_tree(v);
 }
 
+/* the GPU grid of a reduction is a multiple of 8 blocks (one share per
+ * XCD); the extra blocks contribute zero partials */
+static int64_t nblocks8(int64_t n) { return ((n + RBLOCK - 1) / RBLOCK + 7) / 8 * 8; }
+
 double ora_dot_tree(const double* a, const double* b, int64_t n) {
-    int64_t nb = (n + RBLOCK - 1) / RBLOCK;
+    int64_t nb = nblocks8(n);
     if (nb == 0) return 0.0;
     double* part = (double*)malloc(sizeof(double) * nb);
 #pragma omp parallel for schedule(static)
@@ -238,7 +242,7 @@ double ora_dot_tree(const double* a, const double* b, int64_t n) {
 
 /* same tree over float vectors, products formed in double (exact) */
 static double dot_tree_f(const float* a, const float* b, int64_t n) {
-    int64_t nb = (n + RBLOCK - 1) / RBLOCK;
+    int64_t nb = nblocks8(n);
     if (nb == 0) return 0.0;
     double* part = (double*)malloc(sizeof(double) * nb);
 #pragma omp parallel for schedule(static)
@@ -1041,82 +1045,127 @@ int ora_bicgstab_csr(int32_t n, const int32_t* rowptr, const int32_t* col, const
     return rc;
 }
 
-/* Mixed precision: FP32 matrix values and FP32 BiCGSTAB vectors with FP64
- * dot accumulation inside, FP64 residual b - A x and update outside
- * (iterative refinement).  Each inner solve starts from d = 0 on the
- * normalised residual. */
-static int mixed_solve(ora_ctx* c, double* x, int32_t* iters, int32_t* outer,
-                       double* relres) {
-    const isph_params* prm = &c->prm;
-    const int32_t n = c->n;
-    const int32_t nnz = c->rowptr[n];
-    float* val32 = (float*)xmalloc(sizeof(float) * (size_t)nnz);
+static void spmv_csr_fd(int32_t n, const int32_t* rp, const int32_t* col, const float* val,
+                        const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s = s + (double)val[k] * x[col[k]];
+        y[i] = s;
+    }
+}
+
+/* Mixed precision: BiCGSTAB (same recurrences) with FP32 matrix values,
+ * Krylov vectors of type TX (float: ISPH_PREC_MIXED, double:
+ * ISPH_PREC_MIXED_V64), dot products accumulated in FP64, and reliable
+ * updates (Sleijpen & van der Vorst 1996; van der Vorst & Ye 2000): when
+ * the recurrence residual has dropped by delta = inner_tol since the last
+ * update (or below tol), the iterate is added to the FP64 solution, the
+ * FP64 true residual b - A64 x replaces the recurrence residual and the
+ * iteration continues with its search direction kept.  The stop test is the
+ * FP64 true residual; max_outer caps the number of updates. */
+#define MIXED_SOLVE(NAME, TX, SPMV, DOT, VEC)                                                \
+static int NAME(ora_ctx* c, const float* val32, double* x, int32_t* iters, int32_t* outer,  \
+                double* relres) {                                                           \
+    const isph_params* prm = &c->prm;                                                       \
+    const int32_t n = c->n;                                                                 \
+    double* r64 = c->w[6];                                                                  \
+    double* ax = c->w[7];                                                                   \
+    TX *rs = VEC(0), *p = VEC(1), *res = VEC(2), *mm1 = VEC(3), *mm2 = VEC(4), *si = VEC(5), \
+       *sl = VEC(6);                                                                        \
+    const double* b = c->rhs;                                                               \
+    *iters = 0;                                                                             \
+    *outer = 0;                                                                             \
+    int rc = ISPH_OK;                                                                       \
+    spmv_csr(n, c->rowptr, c->col, c->val, x, ax);                                          \
+    for (int32_t i = 0; i < n; ++i) r64[i] = b[i] - ax[i];                                 \
+    double bb = ora_dot_tree(b, b, n);                                                      \
+    double rr = ora_dot_tree(r64, r64, n);                                                  \
+    if (bb == 0.0) {                                                                        \
+        for (int32_t i = 0; i < n; ++i) x[i] = 0.0;                                         \
+        *relres = 0.0;                                                                      \
+        return ISPH_OK;                                                                     \
+    }                                                                                       \
+    const double nb = sqrt(bb);                                                             \
+    *relres = sqrt(rr) / nb;                                                                \
+    if (sqrt(rr) <= prm->tol * nb) return ISPH_OK;                                          \
+    double rupd = sqrt(rr);                                                                 \
+    for (int32_t i = 0; i < n; ++i) {                                                       \
+        rs[i] = (TX)r64[i];                                                                 \
+        res[i] = rs[i];                                                                     \
+        sl[i] = (TX)0;                                                                      \
+    }                                                                                       \
+    double prevr = DOT(res, rs, n), prevr_prev = prevr;                                     \
+    double alpha = 0.0, wi = 0.0, betas = 0.0;                                              \
+    int first = 1;                                                                          \
+    int32_t total = 0, nupd = 0;                                                            \
+    for (;;) {                                                                              \
+        if (total >= prm->maxit) { rc = ISPH_E_NOCONV; break; }                             \
+        total++;                                                                            \
+        if (first) {                                                                        \
+            for (int32_t i = 0; i < n; ++i) p[i] = rs[i];                                   \
+        } else {                                                                            \
+            TX bt = (TX)betas, wt = (TX)wi;                                                 \
+            for (int32_t i = 0; i < n; ++i) p[i] = rs[i] + bt * (p[i] - wt * mm1[i]);       \
+        }                                                                                   \
+        first = 0;                                                                          \
+        SPMV(n, c->rowptr, c->col, val32, p, mm1);                                          \
+        double d1 = DOT(mm1, res, n);                                                       \
+        alpha = prevr / d1;                                                                 \
+        TX at = (TX)alpha;                                                                  \
+        for (int32_t i = 0; i < n; ++i) si[i] = rs[i] - at * mm1[i];                        \
+        SPMV(n, c->rowptr, c->col, val32, si, mm2);                                         \
+        double d2 = DOT(mm2, si, n);                                                        \
+        double d3 = DOT(mm2, mm2, n);                                                       \
+        wi = d2 / d3;                                                                       \
+        TX wt = (TX)wi;                                                                     \
+        for (int32_t i = 0; i < n; ++i) {                                                   \
+            sl[i] = (sl[i] + at * p[i]) + wt * si[i];                                       \
+            rs[i] = si[i] - wt * mm2[i];                                                    \
+        }                                                                                   \
+        double d4 = DOT(rs, res, n);                                                        \
+        double rrk = DOT(rs, rs, n);                                                        \
+        betas = (d4 / prevr) * (alpha / wi);                                                \
+        prevr_prev = prevr;                                                                 \
+        prevr = d4;                                                                         \
+        if (!isfinite(betas) || !isfinite(rrk)) { rc = ISPH_E_NOCONV; break; }              \
+        if (sqrt(rrk) <= prm->tol * nb || sqrt(rrk) <= prm->inner_tol * rupd) {             \
+            for (int32_t i = 0; i < n; ++i) {                                               \
+                x[i] = x[i] + (double)sl[i];                                                \
+                sl[i] = (TX)0;                                                              \
+            }                                                                               \
+            spmv_csr(n, c->rowptr, c->col, c->val, x, ax);                                  \
+            for (int32_t i = 0; i < n; ++i) r64[i] = b[i] - ax[i];                         \
+            double tr = ora_dot_tree(r64, r64, n);                                          \
+            nupd++;                                                                         \
+            *relres = sqrt(tr) / nb;                                                        \
+            if (sqrt(tr) <= prm->tol * nb) break;                                           \
+            if (nupd >= prm->max_outer || total >= prm->maxit) { rc = ISPH_E_NOCONV; break; } \
+            for (int32_t i = 0; i < n; ++i) rs[i] = (TX)r64[i];                             \
+            rupd = sqrt(tr);                                                                \
+            d4 = DOT(rs, res, n);                                                           \
+            betas = (d4 / prevr_prev) * (alpha / wi);                                       \
+            prevr = d4;                                                                     \
+        }                                                                                   \
+    }                                                                                       \
+    *iters = total;                                                                         \
+    *outer = nupd;                                                                          \
+    return rc;                                                                              \
+}
+
+/* work vectors: w[6] = r64, w[7] = A x, w[8] = x are taken */
+#define VEC_F(k) (c->f[k])
+#define VEC_D(k) (c->w[(k) == 6 ? 9 : (k)])
+MIXED_SOLVE(mixed_solve_f32, float, spmv_csr_f, dot_tree_f, VEC_F)
+MIXED_SOLVE(mixed_solve_v64, double, spmv_csr_fd, ora_dot_tree, VEC_D)
+
+static int mixed_solve(ora_ctx* c, double* x, int32_t* iters, int32_t* outer, double* relres) {
+    const int32_t nnz = c->rowptr[c->n];
+    float* val32 = (float*)xmalloc(sizeof(float) * (size_t)(nnz ? nnz : 1));
     for (int32_t k = 0; k < nnz; ++k) val32[k] = (float)c->val[k];
-    double* r64 = c->w[6];
-    double* ax = c->w[7];
-    float *rs = c->f[0], *p = c->f[1], *res = c->f[2], *mm1 = c->f[3], *mm2 = c->f[4],
-          *si = c->f[5], *sl = c->f[6];
-    const double* b = c->rhs;
-    double bb = ora_dot_tree(b, b, n);
-    *iters = 0;
-    *outer = 0;
-    int rc = ISPH_OK;
-    if (bb == 0.0) {
-        for (int32_t i = 0; i < n; ++i) x[i] = 0.0;
-        *relres = 0.0;
-        free(val32);
-        return ISPH_OK;
-    }
-    const double nb = sqrt(bb);
-    int32_t total = 0;
-    for (;;) {
-        spmv_csr(n, c->rowptr, c->col, c->val, x, ax);
-        for (int32_t i = 0; i < n; ++i) r64[i] = b[i] - ax[i];
-        double rr = ora_dot_tree(r64, r64, n);
-        double nr = sqrt(rr);
-        *relres = nr / nb;
-        if (nr <= prm->tol * nb) break;
-        if (*outer >= prm->max_outer || total >= prm->maxit) { rc = ISPH_E_NOCONV; break; }
-        *outer += 1;
-        double inv = 1.0 / nr;
-        for (int32_t i = 0; i < n; ++i) {
-            rs[i] = (float)(r64[i] * inv);
-            p[i] = rs[i];
-            res[i] = rs[i];
-            sl[i] = 0.0f;
-        }
-        double prevr = dot_tree_f(res, rs, n);
-        double rr0 = prevr;
-        int32_t k = 0;
-        while (k < prm->inner_maxit && total < prm->maxit) {
-            k++;
-            total++;
-            spmv_csr_f(n, c->rowptr, c->col, val32, p, mm1);
-            double d1 = dot_tree_f(mm1, res, n);
-            double alpha = prevr / d1;
-            float a32 = (float)alpha;
-            for (int32_t i = 0; i < n; ++i) si[i] = rs[i] - a32 * mm1[i];
-            spmv_csr_f(n, c->rowptr, c->col, val32, si, mm2);
-            double d2 = dot_tree_f(mm2, si, n);
-            double d3 = dot_tree_f(mm2, mm2, n);
-            double wi = d2 / d3;
-            float w32 = (float)wi;
-            for (int32_t i = 0; i < n; ++i) {
-                sl[i] = (sl[i] + a32 * p[i]) + w32 * si[i];
-                rs[i] = si[i] - w32 * mm2[i];
-            }
-            double d4 = dot_tree_f(rs, res, n);
-            double rrk = dot_tree_f(rs, rs, n);
-            double betas = (d4 / prevr) * (alpha / wi);
-            prevr = d4;
-            if (!isfinite(betas) || !isfinite(rrk)) break;
-            if (sqrt(rrk) <= prm->inner_tol * sqrt(rr0)) break;
-            float b32 = (float)betas;
-            for (int32_t i = 0; i < n; ++i) p[i] = rs[i] + b32 * (p[i] - w32 * mm1[i]);
-        }
-        for (int32_t i = 0; i < n; ++i) x[i] = x[i] + (double)sl[i] * nr;
-    }
-    *iters = total;
+    int rc = c->prm.precision == ISPH_PREC_MIXED_V64
+                 ? mixed_solve_v64(c, val32, x, iters, outer, relres)
+                 : mixed_solve_f32(c, val32, x, iters, outer, relres);
     free(val32);
     return rc;
 }
@@ -1133,7 +1182,7 @@ int ora_ppe_solve(ora_ctx* c, int32_t* iters, double* relres) {
     int32_t it = 0, outer = 0;
     double rel = 0.0;
     int rc;
-    if (prm->precision == ISPH_PREC_MIXED) {
+    if (prm->precision == ISPH_PREC_MIXED || prm->precision == ISPH_PREC_MIXED_V64) {
         rc = mixed_solve(c, x, &it, &outer, &rel);
     } else {
         rc = bicgstab_core(c->n, c->rowptr, c->col, c->val, c->rhs, x, prm->tol, prm->maxit,

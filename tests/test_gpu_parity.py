@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from beach_slope_confined_amd import configs
+from beach_slope_confined_amd.params import PREC_MIXED, PREC_MIXED_V64
 
 from parity_util import STAGES, diff_state, pair, run_stage
 
@@ -40,3 +41,31 @@ def test_stage_by_stage_bitwise(require_gpu, case):
                 cg, co = gpu.get_csr(), ora.get_csr()
                 for a, b, what in zip(cg, co, ("rowptr", "col", "val", "rhs")):
                     assert np.array_equal(a, b), f"step {step} csr {what}"
+
+
+@pytest.mark.parametrize("case,prec", [("tank_64", PREC_MIXED), ("tank_64", PREC_MIXED_V64),
+                                       ("dam_break", PREC_MIXED_V64)])
+def test_mixed_precision_bitwise(require_gpu, case, prec):
+    """Mixed-precision PPE (FP32 values, 16-bit offsets, reliable updates):
+    the GPU and the oracle run the same algorithm and agree bit for bit."""
+    gpu, ora = pair(CASES[case](), precision=prec)
+    for step in range(2):
+        gpu.step(1)
+        ora.step(1)
+        assert gpu.scalars()["iters"] == ora.scalars()["iters"], f"step {step} iterations"
+        assert gpu.scalars()["outer"] == ora.scalars()["outer"], f"step {step} updates"
+        assert diff_state(gpu, ora) == [], f"step {step}"
+
+
+def test_mixed_f32_vectors_failure_parity(require_gpu):
+    """With FP32 Krylov vectors the dam-break PPE diverges; the GPU and the
+    oracle report the same failure after the same number of iterations."""
+    from beach_slope_confined_amd.capi import IsphError
+    gpu, ora = pair(CASES["dam_break"](), precision=PREC_MIXED)
+    with pytest.raises(IsphError) as eg:
+        gpu.step(1)
+    with pytest.raises(IsphError) as eo:
+        ora.step(1)
+    assert eg.value.code == eo.value.code == -5
+    assert gpu.scalars()["iters"] == ora.scalars()["iters"]
+    assert gpu.scalars()["outer"] == ora.scalars()["outer"]
