@@ -297,8 +297,9 @@ class IsphSim(Sim):
         self._check(self._lib.isph_restore(self._h, int(slot)), "restore")
         self.n = self.count()
 
-    def profile_enable(self, on=True):
-        self._check(self._lib.isph_profile_enable(self._h, 1 if on else 0), "profile_enable")
+    def profile_enable(self, stride=1):
+        """Time every `stride`-th SpMV launch with HIP events (0: off)."""
+        self._check(self._lib.isph_profile_enable(self._h, int(stride)), "profile_enable")
 
     def kernel_time(self, kernel):
         n = ctypes.c_int64()

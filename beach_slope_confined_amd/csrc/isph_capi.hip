@@ -32,11 +32,16 @@ struct EventPair {
 
 struct Profiler {
     bool on = false;
+    int stride = 1;        // record every stride-th launch of a slot
+    int64_t seen[4] = {0, 0, 0, 0};
+    bool cur[4] = {false, false, false, false};
     std::vector<EventPair> pool[4];
     std::vector<char> real[4];      // per recorded launch: did real work
     size_t used[4] = {0, 0, 0, 0};
     int64_t launches[4] = {0, 0, 0, 0};
     double total_ms[4] = {0, 0, 0, 0};
+    int batch_recorded[4] = {0, 0, 0, 0};
+    int batch_issued[4] = {0, 0, 0, 0};
 };
 
 }  // namespace
@@ -54,7 +59,7 @@ struct isph_ctx {
     int *bincnt = nullptr, *scanbuf = nullptr;
     int *nbr = nullptr, *cnt = nullptr;
     bool have_nbr = false;
-    double *val64 = nullptr, *rhs = nullptr;
+    double *val64 = nullptr, *rhs = nullptr, *rsum = nullptr;
     float* val32 = nullptr;
     int* dpos = nullptr;
     short* nbr16 = nullptr;
@@ -63,6 +68,8 @@ struct isph_ctx {
     bool narrow_idx = false;
     double* v64[12] = {};
     float* v32[10] = {};
+    double2 *rq64 = nullptr, *rv64 = nullptr;
+    float2 *rq32 = nullptr, *rv32 = nullptr;
     double* part = nullptr;
     uint32_t* counters = nullptr;
     SolverScalars* sc = nullptr;
@@ -293,7 +300,7 @@ int run_assemble(isph_ctx* c) {
             c->n, kconsts(c), p.lam, p.rho0 * p.rel_den, (size_t)c->cap, c->st[c->cur], c->nbr,
             c->cnt, c->L, c->val64,
             (p.precision == ISPH_PREC_MIXED || p.precision == ISPH_PREC_MIXED_V64) ? c->val32 : nullptr,
-            c->dpos,
+            p.precision == ISPH_PREC_MIXED_V64 ? c->rsum : nullptr, c->dpos,
             c->rhs, c->ss, c->err);
     HIPCHK(c, hipGetLastError());
     c->assembled = true;
@@ -365,6 +372,8 @@ void prof_record(isph_ctx* c, int k, bool start) {
     Profiler& pr = c->prof;
     if (!pr.on) return;
     if (start) {
+        pr.cur[k] = (pr.seen[k]++ % pr.stride) == 0;
+        if (!pr.cur[k]) return;
         if (pr.used[k] == pr.pool[k].size()) {
             EventPair e;
             (void)hipEventCreate(&e.a);
@@ -374,20 +383,32 @@ void prof_record(isph_ctx* c, int k, bool start) {
         }
         (void)hipEventRecord(pr.pool[k][pr.used[k]].a, c->stream);
     } else {
+        if (!pr.cur[k]) return;
         (void)hipEventRecord(pr.pool[k][pr.used[k]].b, c->stream);
         pr.real[k][pr.used[k]] = 1;
+        pr.batch_recorded[k]++;
         pr.used[k]++;
     }
 }
 
-// Of the `issued` launches of the last batch only the first `real_iters`
-// did work; the rest returned at once because the device had set `done`.
+// Of the launches of the last batch only the first `real_iters` did work;
+// the rest returned at once because the device had set `done`.  Recorded
+// launches are the ones whose index in the slot's sequence is a multiple of
+// the stride, so the recorded launches of the batch are matched to their
+// launch index before the no-op tail is dropped.
 void prof_mark(isph_ctx* c, int k0, int real_iters, int issued) {
     Profiler& pr = c->prof;
     if (!pr.on) return;
     for (int k = k0; k < k0 + 2; ++k) {
-        size_t start = pr.used[k] - (size_t)issued;
-        for (size_t q = start + (size_t)std::max(real_iters, 0); q < pr.used[k]; ++q) pr.real[k][q] = 0;
+        const int rec = pr.batch_recorded[k];
+        const int64_t first_launch = pr.seen[k] - issued;  // index of the batch's first launch
+        size_t q = pr.used[k] - (size_t)rec;
+        for (int64_t l = first_launch; l < pr.seen[k]; ++l) {
+            if (l % pr.stride != 0) continue;
+            if (l - first_launch >= real_iters) pr.real[k][q] = 0;
+            ++q;
+        }
+        pr.batch_recorded[k] = 0;
     }
 }
 
@@ -433,8 +454,10 @@ ResidArgs resid_args(isph_ctx* c, double* x, double* r) {
     return a;
 }
 
+// vector slots: v[0] rhat, v[1] p, v[2] s, v[3] t, v[4] x (or its segment)
 template <class TV, class TX, class TI>
-KrylovArgs<TV, TX, TI> krylov_args(isph_ctx* c, const TI* nbr, const TV* val, TX* const* v) {
+KrylovArgs<TV, TX, TI> krylov_args(isph_ctx* c, const TI* nbr, const TV* val, TX* const* v,
+                                   typename Pair2<TX>::type* rq, typename Pair2<TX>::type* rv) {
     KrylovArgs<TV, TX, TI> a;
     a.n = c->n;
     a.ld = (size_t)c->cap;
@@ -442,14 +465,14 @@ KrylovArgs<TV, TX, TI> krylov_args(isph_ctx* c, const TI* nbr, const TV* val, TX
     a.cnt = c->cnt;
     a.dpos = c->dpos;
     a.val = val;
-    a.rs = v[0];
-    a.rhat = v[1];
-    a.p = v[2];
-    a.q = v[3];
-    a.mm1 = v[4];
-    a.si = v[5];
-    a.mm2 = v[6];
-    a.xs = v[7];
+    a.rsum = c->rsum;
+    a.rq = rq;
+    a.rv = rv;
+    a.rhat = v[0];
+    a.p = v[1];
+    a.si = v[2];
+    a.mm2 = v[3];
+    a.xs = v[4];
     a.part = c->part;
     return a;
 }
@@ -478,13 +501,13 @@ int iterate(isph_ctx* c, const KrylovArgs<TV, TX, TI>& a, const SolveCtl& ctl, b
             else
                 k_krylov_1t<TV, TX, TI, false><<<grid, BLOCK, 0, c->stream>>>(a, c->sc);
             prof_record(c, k0, false);
-            k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_ALPHA, grid, c->part, c->sc, ctl);
+            k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_ALPHA, grid, c->part, c->sc, ctl);
             prof_record(c, k0 + 1, true);
             k_krylov_2<TV, TX, TI><<<grid, BLOCK, 0, c->stream>>>(a, c->sc);
             prof_record(c, k0 + 1, false);
-            k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_OMEGA, grid, c->part, c->sc, ctl);
+            k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_OMEGA, grid, c->part, c->sc, ctl);
             k_krylov_3<TV, TX, TI><<<grid, BLOCK, 0, c->stream>>>(a, c->sc);
-            k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_BETA, grid, c->part, c->sc, ctl);
+            k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_BETA, grid, c->part, c->sc, ctl);
             first = false;
         }
         HIPCHK(c, hipGetLastError());
@@ -497,15 +520,14 @@ int iterate(isph_ctx* c, const KrylovArgs<TV, TX, TI>& a, const SolveCtl& ctl, b
 
 int solve_fp64(isph_ctx* c) {
     const int grid = nblk8(c->n);
-    double* const* v = c->v64;
-    double* x = v[8];
-    // the FP64 solve builds x in place: slot 7 (segment) aliases x
-    double* vv[8] = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], x};
-    auto ka = krylov_args<double, double, int>(c, c->nbr, c->val64, vv);
+    double* x = c->v64[8];
+    // the FP64 solve builds x in place
+    double* vv[5] = {c->v64[0], c->v64[1], c->v64[2], c->v64[3], x};
+    auto ka = krylov_args<double, double, int>(c, c->nbr, c->val64, vv, c->rq64, c->rv64);
     const SolveCtl ctl = solve_ctl(c, 0);
-    ResidArgs ra = resid_args(c, x, v[9]);
-    k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 1, v[0], v[1]);
-    k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, ctl);
+    ResidArgs ra = resid_args(c, x, c->v64[9]);
+    k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 1, c->rq64, c->v64[0]);
+    k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, ctl);
     HIPCHK(c, hipGetLastError());
     int rc = read_sc(c);
     if (rc) return rc;
@@ -519,8 +541,8 @@ int solve_fp64(isph_ctx* c) {
         if (d == DONE_CONVERGED) return ISPH_OK;
         if (d == DONE_FAILED) return fail(c, ISPH_E_NOCONV, "PPE solve did not converge");
         if (d == DONE_RECURRENCE) {
-            k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 1, v[0], v[1]);
-            k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_TRUE, grid, c->part, c->sc, ctl);
+            k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 1, c->rq64, c->v64[0]);
+            k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_TRUE, grid, c->part, c->sc, ctl);
             HIPCHK(c, hipGetLastError());
             if ((rc = read_sc(c))) return rc;
             first = true;
@@ -531,17 +553,18 @@ int solve_fp64(isph_ctx* c) {
 }
 
 template <class TX, class TI>
-int solve_mixed_t(isph_ctx* c, const TI* nbr, TX* const* v, int k0) {
+int solve_mixed_t(isph_ctx* c, const TI* nbr, TX* const* v, typename Pair2<TX>::type* rq,
+                  typename Pair2<TX>::type* rv, int k0) {
     const int grid = nblk8(c->n);
     double* x = c->v64[8];
     double* r64 = c->v64[9];
-    auto ka = krylov_args<float, TX, TI>(c, nbr, c->val32, v);
+    auto ka = krylov_args<float, TX, TI>(c, nbr, c->val32, v, rq, rv);
     SolveCtl ctl = solve_ctl(c, 1);
     SolveCtl ctl_flush = solve_ctl(c, 2);
     ResidArgs ra = resid_args(c, x, r64);
     k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 0, nullptr, nullptr);
-    k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, solve_ctl(c, 0));
-    k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_RESID, grid, c->part, c->sc, ctl);
+    k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, solve_ctl(c, 0));
+    k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_RESID, grid, c->part, c->sc, ctl);
     HIPCHK(c, hipGetLastError());
     int rc = read_sc(c);
     if (rc) return rc;
@@ -550,8 +573,8 @@ int solve_mixed_t(isph_ctx* c, const TI* nbr, TX* const* v, int k0) {
         return ISPH_OK;
     }
     if (c->sc_host->done == DONE_CONVERGED) return ISPH_OK;
-    k_mx_start<TX><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, v[0], v[1], v[7], c->part, c->sc);
-    k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_START, grid, c->part, c->sc, ctl);
+    k_mx_start<TX><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, rq, v[0], v[4], c->part, c->sc);
+    k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_START, grid, c->part, c->sc, ctl);
     HIPCHK(c, hipGetLastError());
     if ((rc = read_sc(c))) return rc;
     bool first = true;
@@ -561,16 +584,16 @@ int solve_mixed_t(isph_ctx* c, const TI* nbr, TX* const* v, int k0) {
         if (d == DONE_FAILED) return fail(c, ISPH_E_NOCONV, "mixed-precision PPE solve did not converge");
         if (d != DONE_UPDATE) return fail(c, ISPH_E_STATE, "unexpected solver state");
         // reliable update: flush x, FP64 true residual, replace r
-        k_mx_flush<TX><<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, x, v[7]);
+        k_mx_flush<TX><<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, x, v[4]);
         k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 0, nullptr, nullptr);
-        k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_RESID, grid, c->part, c->sc, ctl_flush);
+        k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_RESID, grid, c->part, c->sc, ctl_flush);
         HIPCHK(c, hipGetLastError());
         if ((rc = read_sc(c))) return rc;
         if (c->sc_host->done == DONE_CONVERGED) return ISPH_OK;
         if (c->sc_host->done == DONE_FAILED)
             return fail(c, ISPH_E_NOCONV, "mixed-precision PPE solve did not converge");
-        k_mx_replace<TX><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, v[0], v[1], c->part, c->sc);
-        k_finish<<<1, BLOCK, 0, c->stream>>>(FIN_REPLACE, grid, c->part, c->sc, ctl);
+        k_mx_replace<TX><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, rq, v[0], c->part, c->sc);
+        k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_REPLACE, grid, c->part, c->sc, ctl);
         HIPCHK(c, hipGetLastError());
     }
 }
@@ -586,15 +609,13 @@ int solve_mixed(isph_ctx* c) {
     if (!narrow) HIPCHK(c, hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->stream));
     c->narrow_idx = narrow;
     if (c->prm.precision == ISPH_PREC_MIXED_V64) {
-        double* vv[8] = {c->v64[0], c->v64[1], c->v64[2], c->v64[3],
-                         c->v64[4], c->v64[5], c->v64[6], c->v64[7]};
-        return narrow ? solve_mixed_t<double, short>(c, c->nbr16, vv, 2)
-                      : solve_mixed_t<double, int>(c, c->nbr, vv, 2);
+        double* vv[5] = {c->v64[0], c->v64[1], c->v64[2], c->v64[3], c->v64[4]};
+        return narrow ? solve_mixed_t<double, short>(c, c->nbr16, vv, c->rq64, c->rv64, 2)
+                      : solve_mixed_t<double, int>(c, c->nbr, vv, c->rq64, c->rv64, 2);
     }
-    float* vv[8] = {c->v32[0], c->v32[1], c->v32[2], c->v32[3],
-                    c->v32[4], c->v32[5], c->v32[6], c->v32[7]};
-    return narrow ? solve_mixed_t<float, short>(c, c->nbr16, vv, 2)
-                  : solve_mixed_t<float, int>(c, c->nbr, vv, 2);
+    float* vv[5] = {c->v32[0], c->v32[1], c->v32[2], c->v32[3], c->v32[4]};
+    return narrow ? solve_mixed_t<float, short>(c, c->nbr16, vv, c->rq32, c->rv32, 2)
+                  : solve_mixed_t<float, int>(c, c->nbr, vv, c->rq32, c->rv32, 2);
 }
 
 int run_solve(isph_ctx* c) {
@@ -684,8 +705,13 @@ int isph_create(const isph_params* prm, int32_t cap, int32_t cap_soil, isph_ctx*
     rc |= dalloc(c, &c->dpos, N);
     rc |= dalloc(c, &c->nbr16, N * M);
     rc |= dalloc(c, &c->rhs, N);
+    rc |= dalloc(c, &c->rsum, N);
     for (int k = 0; k < 10; ++k) rc |= dalloc(c, &c->v64[k], N);
     for (int k = 0; k < 9; ++k) rc |= dalloc(c, &c->v32[k], N);
+    rc |= dalloc(c, &c->rq64, N);
+    rc |= dalloc(c, &c->rv64, N);
+    rc |= dalloc(c, &c->rq32, N);
+    rc |= dalloc(c, &c->rv32, N);
     rc |= dalloc(c, &c->part, 4 * (size_t)nblk8(cap));
     rc |= dalloc(c, &c->counters, 16);
     rc |= dalloc(c, &c->sc, 1);
@@ -718,6 +744,7 @@ int isph_destroy(isph_ctx* c) {
     free_state(c->st[0]);
     free_state(c->st[1]);
     void* p[] = {c->L, c->divr, c->key, c->key_s, c->rank, c->perm, c->bincnt, c->scanbuf, c->nbr16,
+                 c->rq64, c->rv64, c->rq32, c->rv32, c->rsum,
                  c->nbr, c->cnt, c->val64, c->val32, c->dpos, c->rhs, c->part, c->counters,
                  c->sc, c->ss, c->err, c->spos, c->smass, c->srho, c->sstart};
     for (auto q : p)
@@ -1150,12 +1177,15 @@ int isph_restore(isph_ctx* c, int32_t slot) {
 }
 
 int isph_profile_enable(isph_ctx* c, int32_t enable) {
-    if (!c) return ISPH_E_ARG;
+    if (!c || enable < 0) return ISPH_E_ARG;
     c->prof.on = enable != 0;
+    c->prof.stride = enable > 0 ? enable : 1;
     for (int k = 0; k < 4; ++k) {
         c->prof.launches[k] = 0;
         c->prof.total_ms[k] = 0.0;
         c->prof.used[k] = 0;
+        c->prof.seen[k] = 0;
+        c->prof.batch_recorded[k] = 0;
     }
     return ISPH_OK;
 }

@@ -92,19 +92,47 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks) {
     return (bid & 7) * per + (bid >> 3);
 }
 
+template <class T> struct Pair2;
+template <> struct Pair2<double> { using type = double2; };
+template <> struct Pair2<float> { using type = float2; };
+
 // Operands of one BiCGSTAB flavour: TV matrix values, TX vectors, TI column
-// index (int: absolute, short: offset from the row).
+// index (int: absolute, short: offset from the row).  The two gathered
+// operands are stored as pairs so that a neighbour costs one load:
+// rq = (r, q) for the p update of step 1, rv = (r, v) for s = r - alpha v
+// of step 2.
 template <class TV, class TX, class TI>
 struct KrylovArgs {
+    using P2 = typename Pair2<TX>::type;
     int n;
     size_t ld;
     const TI* nbr;
     const int* cnt;
     const int* dpos;
     const TV* val;
-    TX *rs, *rhat, *p, *q, *mm1, *si, *mm2, *xs;
+    const double* rsum;  // FP64 row sums (conservation-form SpMV of MIXED_V64)
+    P2 *rq, *rv;
+    TX *rhat, *p, *si, *mm2, *xs;
     double* part;  // 2 * nbp block partials
 };
+
+constexpr int FIN_BLOCK = 1024;  // threads of the one-block reduction kernel
+
+// block sum over FIN_BLOCK threads: wave butterflies, then the 16 wave sums
+// as a balanced pairwise tree; valid in every thread
+__device__ __forceinline__ double block_sum_fin(double v) {
+    __shared__ double red[FIN_BLOCK / 64];
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = red[2 * k] + red[2 * k + 1];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = a[2 * k] + a[2 * k + 1];
+    return ((a[0] + a[1]) + (a[2] + a[3]));
+}
 
 // Operands of an FP64 residual b - A64 x.
 struct ResidArgs {
@@ -164,7 +192,7 @@ __global__ void k_vstar(int n, KernelConsts kc, PorousParams pp, double lam, dou
                         const SimScalars* ss);
 __global__ void k_assemble(int n, KernelConsts kc, double lam, double lampd, size_t ld,
                            StateView s, const int* nbr, const int* cnt, const double* L,
-                           double* val, float* val32, int* dpos, double* rhs,
+                           double* val, float* val32, double* rsum, int* dpos, double* rhs,
                            const SimScalars* ss, uint32_t* err);
 template <class TV, class TX, class TI, bool FIRST>
 __global__ void k_krylov_1t(KrylovArgs<TV, TX, TI> a, const SolverScalars* sc);
@@ -173,13 +201,13 @@ __global__ void k_krylov_2(KrylovArgs<TV, TX, TI> a, const SolverScalars* sc);
 template <class TV, class TX, class TI>
 __global__ void k_krylov_3(KrylovArgs<TV, TX, TI> a, const SolverScalars* sc);
 __global__ void k_finish(int kind, int nbp, const double* part, SolverScalars* sc, SolveCtl ctl);
-__global__ void k_resid(ResidArgs a, int mode, double* rs, double* rhat);
+__global__ void k_resid(ResidArgs a, int mode, double2* rq, double* rhat);
 template <class TX>
-__global__ void k_mx_start(int n, const double* r64, TX* rs, TX* rhat, TX* xs, double* part,
-                           const SolverScalars* sc);
+__global__ void k_mx_start(int n, const double* r64, typename Pair2<TX>::type* rq, TX* rhat,
+                           TX* xs, double* part, const SolverScalars* sc);
 template <class TX>
-__global__ void k_mx_replace(int n, const double* r64, TX* rs, const TX* rhat, double* part,
-                             const SolverScalars* sc);
+__global__ void k_mx_replace(int n, const double* r64, typename Pair2<TX>::type* rq,
+                             const TX* rhat, double* part, const SolverScalars* sc);
 template <class TX>
 __global__ void k_mx_flush(int n, double* x64, TX* xs);
 __global__ void k_to_short(int n, size_t ld, const int* nbr, const int* cnt, short* out,

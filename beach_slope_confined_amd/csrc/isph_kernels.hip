@@ -440,6 +440,7 @@ __global__ void __launch_bounds__(BLOCK) k_assemble(int n, KernelConsts kc, doub
                                                     const double* __restrict__ L,
                                                     double* __restrict__ val,
                                                     float* __restrict__ val32,
+                                                    double* __restrict__ rsum,
                                                     int* __restrict__ dpos, double* __restrict__ rhs,
                                                     const SimScalars* ss, uint32_t* err) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -521,11 +522,16 @@ __global__ void __launch_bounds__(BLOCK) k_assemble(int n, KernelConsts kc, doub
         val[(size_t)k * ld + i] = v;
     }
     if (!(dia != 0.0) || !isfinite(dia)) raise_err(err, ERR_SINGULAR);
+    double rs = 0.0;  // row sum of the scaled row, columns in order, diagonal 1
     for (int k = 0; k < c; ++k) {
         double v = val[(size_t)k * ld + i] / dia;
         val[(size_t)k * ld + i] = v;
         if (val32) val32[(size_t)k * ld + i] = (float)v;
+        if (k == dp) rs = rs + 1.0;
+        rs = rs + v;
     }
+    if (dp == c) rs = rs + 1.0;
+    if (rsum) rsum[i] = rs;
     dpos[i] = dp;
     rhs[i] = b / dia;
 }

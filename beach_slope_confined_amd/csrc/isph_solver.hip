@@ -9,19 +9,23 @@
 //   k_krylov_3  x += alpha p + omega s, r = s - omega t, q = p - omega v,
 //               (r, rhat), (r, r)
 // q carries p - omega v from one iteration to the next so that the p update
-// costs two gathers (r_j, q_j) instead of three, with the same rounding as
-// r + beta (p - omega v).  Partials are plain stores; the kernel boundary
+// needs (r_j, q_j) instead of three vectors, with the same rounding as
+// r + beta (p - omega v); (r, q) and (r, v) are stored as pairs so that each
+// neighbour is one gather.  Partials are plain stores; the kernel boundary
 // makes them visible, so no fences or global atomics are needed.
 //
 // Flavours (template TV matrix value, TX vector, TI column index):
 //   <double, double, int>    FP64 drop-in
 //   <float,  float,  short>  mixed: FP32 values and FP32 Krylov vectors
-//   <float,  double, short>  mixed: FP32 values, FP64 Krylov vectors
+//   <float,  double, short>  mixed: FP32 values, FP64 Krylov vectors,
+//                            conservation-form SpMV with FP64 row sums
 // short indices are offsets j - i (built only when every offset fits).  The
 // mixed flavours add reliable updates: when the recurrence residual has
 // dropped by `delta`, x is flushed into the FP64 solution and the FP64 true
 // residual b - A64 x replaces the recurrence residual (search direction
 // kept).  Dot products always accumulate in FP64.
+#include <type_traits>
+
 #include "isph_kernels.h"
 
 namespace isph {
@@ -64,22 +68,67 @@ __device__ __forceinline__ TX spmv_row(int i, int c, int dp, const TI* __restric
     return s;
 }
 
+// Conservation form for FP32 values with FP64 vectors (MIXED_V64):
+// (A x)_i = s_i x_i + sum_j a_ij (x_j - x_i) with the FP64 row sum s_i.  The
+// FP32 rounding of a_ij then multiplies x_j - x_i, which is small for the
+// smooth modes whose eigenvalues are small, instead of x_j itself.
+template <class TV, class TX, class TI, class F>
+__device__ __forceinline__ TX spmv_row_diff(int i, int c, double si, const TI* __restrict__ nbr,
+                                            const TV* __restrict__ val, size_t ld, TX xi, F&& xj) {
+    TX acc = (TX)0;
+    int k = 0;
+    for (; k + 4 <= c; k += 4) {
+        const size_t o = (size_t)k * ld + i;
+        const int j0 = col_of(nbr[o], i), j1 = col_of(nbr[o + ld], i);
+        const int j2 = col_of(nbr[o + 2 * ld], i), j3 = col_of(nbr[o + 3 * ld], i);
+        const TX v0 = (TX)val[o], v1 = (TX)val[o + ld], v2 = (TX)val[o + 2 * ld],
+                 v3 = (TX)val[o + 3 * ld];
+        const TX x0 = xj(j0), x1 = xj(j1), x2 = xj(j2), x3 = xj(j3);
+        acc = acc + v0 * (x0 - xi);
+        acc = acc + v1 * (x1 - xi);
+        acc = acc + v2 * (x2 - xi);
+        acc = acc + v3 * (x3 - xi);
+    }
+    for (; k < c; ++k) {
+        const size_t o = (size_t)k * ld + i;
+        const int j = col_of(nbr[o], i);
+        const TX v = (TX)val[o];
+        acc = acc + v * (xj(j) - xi);
+    }
+    return (TX)si * xi + acc;
+}
+
+template <class TV, class TX, class TI, class F>
+__device__ __forceinline__ TX spmv_any(const KrylovArgs<TV, TX, TI>& a, int i, TX xi, F&& xj) {
+    if constexpr (std::is_same<TV, float>::value && std::is_same<TX, double>::value)
+        return spmv_row_diff<TV, TX, TI>(i, a.cnt[i], a.rsum[i], a.nbr, a.val, a.ld, xi, xj);
+    else
+        return spmv_row<TV, TX, TI>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, xi, xj);
+}
+
 template <class TV, class TX, class TI, bool FIRST>
 __global__ void __launch_bounds__(BLOCK) k_krylov_1t(KrylovArgs<TV, TX, TI> a,
                                                      const SolverScalars* sc) {
+    using P2 = typename Pair2<TX>::type;
     if (sc->done) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int i = lb * BLOCK + threadIdx.x;
     const TX beta = (TX)sc->betas;
-    const TX* __restrict__ rs = a.rs;
-    const TX* __restrict__ q = a.q;
-    auto pn = [&](int j) -> TX { return FIRST ? rs[j] : rs[j] + beta * q[j]; };
+    const P2* __restrict__ rq = a.rq;
+    auto pn = [&](int j) -> TX {
+        const P2 u = rq[j];
+        return FIRST ? u.x : u.x + beta * u.y;
+    };
     double v = 0.0;
     if (i < a.n) {
-        const TX pi = pn(i);
-        const TX s = spmv_row<TV, TX, TI>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, pi, pn);
+        const P2 ui = rq[i];
+        const TX pi = FIRST ? ui.x : ui.x + beta * ui.y;
+        const TX s = spmv_any(a, i, pi, pn);
         a.p[i] = pi;
-        a.mm1[i] = s;
+        P2 w;
+        w.x = ui.x;
+        w.y = s;
+        a.rv[i] = w;
         v = (double)s * (double)a.rhat[i];
     }
     v = block_sum(v);
@@ -89,17 +138,20 @@ __global__ void __launch_bounds__(BLOCK) k_krylov_1t(KrylovArgs<TV, TX, TI> a,
 template <class TV, class TX, class TI>
 __global__ void __launch_bounds__(BLOCK) k_krylov_2(KrylovArgs<TV, TX, TI> a,
                                                     const SolverScalars* sc) {
+    using P2 = typename Pair2<TX>::type;
     if (sc->done) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int i = lb * BLOCK + threadIdx.x;
     const TX alpha = (TX)sc->alpha;
-    const TX* __restrict__ rs = a.rs;
-    const TX* __restrict__ mm1 = a.mm1;
-    auto sf = [&](int j) -> TX { return rs[j] - alpha * mm1[j]; };
+    const P2* __restrict__ rv = a.rv;
+    auto sf = [&](int j) -> TX {
+        const P2 u = rv[j];
+        return u.x - alpha * u.y;
+    };
     double v0 = 0.0, v1 = 0.0;
     if (i < a.n) {
         const TX si = sf(i);
-        const TX s = spmv_row<TV, TX, TI>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld, si, sf);
+        const TX s = spmv_any(a, i, si, sf);
         a.si[i] = si;
         a.mm2[i] = s;
         v0 = (double)s * (double)si;
@@ -116,6 +168,7 @@ __global__ void __launch_bounds__(BLOCK) k_krylov_2(KrylovArgs<TV, TX, TI> a,
 template <class TV, class TX, class TI>
 __global__ void __launch_bounds__(BLOCK) k_krylov_3(KrylovArgs<TV, TX, TI> a,
                                                     const SolverScalars* sc) {
+    using P2 = typename Pair2<TX>::type;
     if (sc->done) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int i = lb * BLOCK + threadIdx.x;
@@ -123,12 +176,15 @@ __global__ void __launch_bounds__(BLOCK) k_krylov_3(KrylovArgs<TV, TX, TI> a,
     double v0 = 0.0, v1 = 0.0;
     if (i < a.n) {
         const TX si = a.si[i], p = a.p[i];
+        const TX mm1 = a.rv[i].y;
         const TX x = (a.xs[i] + alpha * p) + w * si;
         const TX r = si - w * a.mm2[i];
-        const TX qq = p - w * a.mm1[i];
+        const TX qq = p - w * mm1;
         a.xs[i] = x;
-        a.rs[i] = r;
-        a.q[i] = qq;
+        P2 u;
+        u.x = r;
+        u.y = qq;
+        a.rq[i] = u;
         v0 = (double)r * (double)a.rhat[i];
         v1 = (double)r * (double)r;
     }
@@ -140,20 +196,35 @@ __global__ void __launch_bounds__(BLOCK) k_krylov_3(KrylovArgs<TV, TX, TI> a,
     }
 }
 
-// One block: finish the reductions of the preceding row kernel and update
-// the solver scalars.  Thread t sums partials t, t+256, ... in order, then
-// the block tree.
-__global__ void __launch_bounds__(BLOCK) k_finish(int kind, int nbp, const double* __restrict__ part,
-                                                  SolverScalars* sc, SolveCtl ctl) {
+// One block of FIN_BLOCK threads: finish the reductions of the preceding row
+// kernel and update the solver scalars.  Thread t sums partials t,
+// t+FIN_BLOCK, ... in order, then the block tree.
+__global__ void __launch_bounds__(FIN_BLOCK) k_finish(int kind, int nbp,
+                                                      const double* __restrict__ part,
+                                                      SolverScalars* sc, SolveCtl ctl) {
     if ((kind == FIN_ALPHA || kind == FIN_OMEGA || kind == FIN_BETA) && sc->done) return;
     const bool two = kind != FIN_ALPHA && kind != FIN_START && kind != FIN_REPLACE;
+    double p0[8], p1[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int k = threadIdx.x + u * FIN_BLOCK;
+        p0[u] = k < nbp ? part[k] : 0.0;
+        p1[u] = (two && k < nbp) ? part[nbp + k] : 0.0;
+    }
     double s0 = 0.0, s1 = 0.0;
-    for (int k = threadIdx.x; k < nbp; k += BLOCK) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        if (threadIdx.x + u * FIN_BLOCK < nbp) {
+            s0 = s0 + p0[u];
+            s1 = s1 + p1[u];
+        }
+    }
+    for (int k = threadIdx.x + 8 * FIN_BLOCK; k < nbp; k += FIN_BLOCK) {
         s0 = s0 + part[k];
         if (two) s1 = s1 + part[nbp + k];
     }
-    const double t0 = block_sum(s0);
-    const double t1 = two ? block_sum(s1) : 0.0;
+    const double t0 = block_sum_fin(s0);
+    const double t1 = two ? block_sum_fin(s1) : 0.0;
     if (threadIdx.x != 0) return;
     switch (kind) {
         case FIN_INIT: {
@@ -270,7 +341,7 @@ __global__ void __launch_bounds__(BLOCK) k_finish(int kind, int nbp, const doubl
 
 // r = b - A64 x in FP64; partials (b,b), (r,r).  mode 1 also restarts the
 // FP64 recurrence from r (rs = rhat = r).
-__global__ void __launch_bounds__(BLOCK) k_resid(ResidArgs a, int mode, double* __restrict__ rs,
+__global__ void __launch_bounds__(BLOCK) k_resid(ResidArgs a, int mode, double2* __restrict__ rq,
                                                  double* __restrict__ rhat) {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int i = lb * BLOCK + threadIdx.x;
@@ -283,7 +354,7 @@ __global__ void __launch_bounds__(BLOCK) k_resid(ResidArgs a, int mode, double* 
         const double r = bi - s;
         a.r[i] = r;
         if (mode == 1) {
-            rs[i] = r;
+            rq[i] = make_double2(r, 0.0);
             rhat[i] = r;
         }
         vb = bi * bi;
@@ -299,15 +370,19 @@ __global__ void __launch_bounds__(BLOCK) k_resid(ResidArgs a, int mode, double* 
 
 template <class TX>
 __global__ void __launch_bounds__(BLOCK) k_mx_start(int n, const double* __restrict__ r64,
-                                                    TX* __restrict__ rs, TX* __restrict__ rhat,
-                                                    TX* __restrict__ xs, double* __restrict__ part,
+                                                    typename Pair2<TX>::type* __restrict__ rq,
+                                                    TX* __restrict__ rhat, TX* __restrict__ xs,
+                                                    double* __restrict__ part,
                                                     const SolverScalars* sc) {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int i = lb * BLOCK + threadIdx.x;
     double v = 0.0;
     if (i < n) {
         const TX r = (TX)r64[i];
-        rs[i] = r;
+        typename Pair2<TX>::type u;
+        u.x = r;
+        u.y = (TX)0;
+        rq[i] = u;
         rhat[i] = r;
         xs[i] = (TX)0;
         v = (double)r * (double)r;
@@ -318,7 +393,7 @@ __global__ void __launch_bounds__(BLOCK) k_mx_start(int n, const double* __restr
 
 template <class TX>
 __global__ void __launch_bounds__(BLOCK) k_mx_replace(int n, const double* __restrict__ r64,
-                                                      TX* __restrict__ rs,
+                                                      typename Pair2<TX>::type* __restrict__ rq,
                                                       const TX* __restrict__ rhat,
                                                       double* __restrict__ part,
                                                       const SolverScalars* sc) {
@@ -327,7 +402,7 @@ __global__ void __launch_bounds__(BLOCK) k_mx_replace(int n, const double* __res
     double v = 0.0;
     if (i < n) {
         const TX r = (TX)r64[i];
-        rs[i] = r;
+        rq[i].x = r;
         v = (double)r * (double)rhat[i];
     }
     v = block_sum(v);
@@ -379,13 +454,13 @@ ISPH_MIXED_FLAVOUR(float, short)
 ISPH_MIXED_FLAVOUR(float, int)
 ISPH_MIXED_FLAVOUR(double, short)
 ISPH_MIXED_FLAVOUR(double, int)
-template __global__ void k_mx_start<float>(int, const double*, float*, float*, float*, double*,
+template __global__ void k_mx_start<float>(int, const double*, float2*, float*, float*, double*,
                                            const SolverScalars*);
-template __global__ void k_mx_start<double>(int, const double*, double*, double*, double*, double*,
-                                            const SolverScalars*);
-template __global__ void k_mx_replace<float>(int, const double*, float*, const float*, double*,
+template __global__ void k_mx_start<double>(int, const double*, double2*, double*, double*,
+                                            double*, const SolverScalars*);
+template __global__ void k_mx_replace<float>(int, const double*, float2*, const float*, double*,
                                              const SolverScalars*);
-template __global__ void k_mx_replace<double>(int, const double*, double*, const double*, double*,
+template __global__ void k_mx_replace<double>(int, const double*, double2*, const double*, double*,
                                               const SolverScalars*);
 template __global__ void k_mx_flush<float>(int, double*, float*);
 template __global__ void k_mx_flush<double>(int, double*, double*);

@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ISPH particle-steps/s @1M (FP64 vs mixed-prec PPE); adjoint/forward cost"
 HBM_PEAK_GBS = 8000.0
+PROFILE_STRIDE = 16  # HIP events around every 16th SpMV launch of the timed steps
 
 
 def parse():
@@ -44,6 +45,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=400,
                     help="BiCGSTAB iterations timed on the CPU oracle")
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="short run for hardware-counter profiles: one step whose FP64 solve is "
+                         "capped at 40 iterations, then 5 copies of 1 GiB with the library's "
+                         "copy kernel (counter calibration)")
     return ap.parse_args()
 
 
@@ -62,7 +67,7 @@ def kernel_bytes(kind, nnz_off, n, val_bytes, vec_bytes):
 
 def run_gpu(cfg, steps, warmup, precision, profile):
     from beach_slope_confined_amd import capi, configs
-    from beach_slope_confined_amd.params import PREC_MIXED
+    from beach_slope_confined_amd.params import PREC_FP64
     c = dict(cfg)
     c["params"] = cfg["params"].copy(precision=precision)
     sim = configs.make_sim(c, capi.IsphSim)
@@ -78,7 +83,7 @@ def run_gpu(cfg, steps, warmup, precision, profile):
         if first is None:
             first = sim.get_particles()
     if profile:
-        sim.profile_enable(True)
+        sim.profile_enable(PROFILE_STRIDE)
     import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -93,7 +98,7 @@ def run_gpu(cfg, steps, warmup, precision, profile):
     out = dict(sim=sim, seconds=t1 - t0, iters=iters, relres=s["relres"], outer=s["outer"],
                nnz_off=int(ptr[-1]), n=len(ptr) - 1, first=first)
     if profile:
-        k0, k1 = (2, 3) if precision == PREC_MIXED else (0, 1)
+        k0, k1 = (0, 1) if precision == PREC_FP64 else (2, 3)
         out["ktime"] = {"p_spmv": sim.kernel_time(k0), "s_spmv": sim.kernel_time(k1)}
     return out
 
@@ -152,6 +157,19 @@ def main():
     from beach_slope_confined_amd.params import PREC_FP64, PREC_MIXED, PREC_MIXED_V64
 
     cfg = configs.tank_1m(args.nside)
+    if args.pmc_run:
+        from beach_slope_confined_amd import capi
+        c = dict(cfg)
+        c["params"] = cfg["params"].copy(maxit=40)
+        sim = configs.make_sim(c, capi.IsphSim)
+        try:
+            sim.step(1)
+        except capi.IsphError as e:
+            print("solve capped as intended:", e)
+        ms = sim.copy_bench(1 << 30, 5)
+        print(json.dumps({"pmc_run": True, "copy_bytes": 1 << 30, "copy_ms": ms,
+                          "copy_GBs": 2 * (1 << 30) / ms / 1e6}))
+        return
     n_fluid = cfg["n_fluid"]
     n_total = len(cfg["parts"]["x"])
 

@@ -212,10 +212,12 @@ int isph_checkpoint(isph_ctx* ctx, int32_t slot);
 int isph_restore(isph_ctx* ctx, int32_t slot);
 
 /* ---- instrumentation (no reference counterpart) -------------------------- */
-/* When enabled, HIP events bracket every launch of the BiCGSTAB SpMV kernels
- * on the context stream; isph_get_kernel_time() returns launches and total
- * milliseconds since the last reset.  kernel: 0 = FP64 p-update+SpMV,
- * 1 = FP64 s+SpMV, 2 = FP32 p-update+SpMV, 3 = FP32 s+SpMV. */
+/* enable = s > 0: HIP events bracket every s-th launch of each BiCGSTAB SpMV
+ * kernel on the context stream (0 turns it off); isph_get_kernel_time()
+ * returns the timed launches and their total milliseconds since the last
+ * enable.  kernel: 0 = FP64 step-1 SpMV (p update, v = A p), 1 = FP64
+ * step-2 SpMV (s, t = A s), 2 and 3 = the same for the mixed-precision
+ * flavours. */
 int isph_profile_enable(isph_ctx* ctx, int32_t enable);
 int isph_get_kernel_time(isph_ctx* ctx, int32_t kernel, int64_t* launches,
                          double* total_ms);

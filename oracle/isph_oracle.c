@@ -205,17 +205,27 @@ static double block_tree(double* v /* RBLOCK, destroyed */) {
     return (w[0] + w[1]) + (w[2] + w[3]);
 }
 
-/* Block partials of a[i]*b[i] over blocks of RBLOCK rows, then the last
- * block's reduction: thread t sums partials t, t+256, ... in order, then
- * the block tree. */
+/* The block partials are finished by one block of FBLOCK threads: thread t
+ * sums partials t, t+FBLOCK, ... in order, then each wave's butterfly, then
+ * the 16 wave sums as a balanced pairwise tree. */
+#define FBLOCK 1024
 static double tree_finish(const double* part, int64_t nb) {
-    double v[RBLOCK];
-    for (int t = 0; t < RBLOCK; ++t) {
+    static double v[FBLOCK];
+    double w[FBLOCK / 64];
+    for (int t = 0; t < FBLOCK; ++t) {
         double s = 0.0;
-        for (int64_t k = t; k < nb; k += RBLOCK) s += part[k];
+        for (int64_t k = t; k < nb; k += FBLOCK) s += part[k];
         v[t] = s;
     }
-    return block_tree(v);
+    for (int k = 0; k < FBLOCK / 64; ++k) {
+        double* a = v + 64 * k;
+        for (int st = 1; st < 64; st <<= 1)
+            for (int l = 0; l < 64; l += 2 * st) a[l] = a[l] + a[l + st];
+        w[k] = a[0];
+    }
+    for (int st = 1; st < FBLOCK / 64; st <<= 1)
+        for (int l = 0; l < FBLOCK / 64; l += 2 * st) w[l] = w[l] + w[l + st];
+    return w[0];
 }
 
 /* the GPU grid of a reduction is a multiple of 8 blocks (one share per
@@ -1045,13 +1055,21 @@ int ora_bicgstab_csr(int32_t n, const int32_t* rowptr, const int32_t* col, const
     return rc;
 }
 
+/* FP32 values, FP64 vectors, conservation form: y_i = s_i x_i +
+ * sum_{j != i} a_ij (x_j - x_i) in column order, s_i the FP64 row sum of
+ * the scaled row (columns in order, diagonal included). */
+static const double* g_rsum;
 static void spmv_csr_fd(int32_t n, const int32_t* rp, const int32_t* col, const float* val,
                         const double* x, double* y) {
 #pragma omp parallel for schedule(static)
     for (int32_t i = 0; i < n; ++i) {
-        double s = 0.0;
-        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s = s + (double)val[k] * x[col[k]];
-        y[i] = s;
+        double acc = 0.0;
+        const double xi = x[i];
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            if (col[k] == i) continue;
+            acc = acc + (double)val[k] * (x[col[k]] - xi);
+        }
+        y[i] = g_rsum[i] * xi + acc;
     }
 }
 
@@ -1163,10 +1181,19 @@ static int mixed_solve(ora_ctx* c, double* x, int32_t* iters, int32_t* outer, do
     const int32_t nnz = c->rowptr[c->n];
     float* val32 = (float*)xmalloc(sizeof(float) * (size_t)(nnz ? nnz : 1));
     for (int32_t k = 0; k < nnz; ++k) val32[k] = (float)c->val[k];
+    double* rsum = (double*)xmalloc(sizeof(double) * (size_t)(c->n ? c->n : 1));
+    for (int32_t i = 0; i < c->n; ++i) {
+        double s = 0.0;
+        for (int32_t k = c->rowptr[i]; k < c->rowptr[i + 1]; ++k) s = s + c->val[k];
+        rsum[i] = s;
+    }
+    g_rsum = rsum;
     int rc = c->prm.precision == ISPH_PREC_MIXED_V64
                  ? mixed_solve_v64(c, val32, x, iters, outer, relres)
                  : mixed_solve_f32(c, val32, x, iters, outer, relres);
     free(val32);
+    free(rsum);
+    g_rsum = NULL;
     return rc;
 }
 
