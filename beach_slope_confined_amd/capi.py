@@ -32,6 +32,36 @@ SUBROUTINES = [
 ]
 
 
+OBJ_MAX = 8
+
+# stage ids of isph_adjoint_stage (include/isph_capi.h)
+ADJ_STAGE = {
+    "projection": 1, "neighbour": 3, "effpor": 4, "eddyvis": 5, "varts": 6, "vstar": 7,
+    "ppesolve": 8, "comp_vel": 10, "comp_pos": 11, "shift": 16, "objective": 20,
+}
+
+
+class IsphObjective(ctypes.Structure):
+    """isph_objective: Shepard pressure probes and water-depth gauges."""
+    _fields_ = [("nprobe", ctypes.c_int32), ("ngauge", ctypes.c_int32),
+                ("probe_x", ctypes.c_double * OBJ_MAX), ("probe_y", ctypes.c_double * OBJ_MAX),
+                ("gauge_x", ctypes.c_double * OBJ_MAX)]
+
+
+def make_objective(probes=(), gauges=()):
+    """probes: sequence of (x, y); gauges: sequence of x."""
+    o = IsphObjective()
+    probes, gauges = list(probes), list(gauges)
+    if len(probes) > OBJ_MAX or len(gauges) > OBJ_MAX:
+        raise ValueError(f"at most {OBJ_MAX} probes and {OBJ_MAX} gauges")
+    o.nprobe, o.ngauge = len(probes), len(gauges)
+    for k, (x, y) in enumerate(probes):
+        o.probe_x[k], o.probe_y[k] = x, y
+    for k, x in enumerate(gauges):
+        o.gauge_x[k] = x
+    return o
+
+
 class IsphError(RuntimeError):
     def __init__(self, code, where, msg):
         self.code = code
@@ -103,6 +133,18 @@ def load_library(path=LIB_PATH):
             ("copy_bench", [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, _c_dp], ctypes.c_int),
             ("checkpoint", [ctypes.c_void_p, ctypes.c_int32], ctypes.c_int),
             ("restore", [ctypes.c_void_p, ctypes.c_int32], ctypes.c_int),
+            ("adjoint_forward", [ctypes.c_void_p, ctypes.POINTER(IsphObjective), ctypes.c_int32,
+                                 ctypes.c_int32, _c_dp], ctypes.c_int),
+            ("adjoint_backward", [ctypes.c_void_p, _c_dp, _c_dp, ctypes.POINTER(ctypes.c_int64)],
+             ctypes.c_int),
+            ("adjoint_stage", [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(IsphObjective)],
+             ctypes.c_int),
+            ("adjoint_zero", [ctypes.c_void_p], ctypes.c_int),
+            ("adjoint_get", [ctypes.c_void_p, ctypes.c_int32, _c_dp], ctypes.c_int),
+            ("adjoint_set", [ctypes.c_void_p, ctypes.c_int32, _c_dp], ctypes.c_int),
+            ("adjoint_params", [ctypes.c_void_p, _c_dp, _c_dp], ctypes.c_int),
+            ("objective_eval", [ctypes.c_void_p, ctypes.POINTER(IsphObjective), _c_dp],
+             ctypes.c_int),
         ]:
             f = getattr(lib, "isph_" + name)
             f.argtypes = args
@@ -313,6 +355,59 @@ class IsphSim(Sim):
         self._check(self._lib.isph_copy_bench(self._h, int(nbytes), int(reps), ctypes.byref(ms)),
                     "copy_bench")
         return ms.value
+
+    # -- discrete adjoint ---------------------------------------------------
+    def adjoint_forward(self, obj, nsteps, every=10):
+        """Porous closure + timestep at the current state, then nsteps steps
+        with checkpoints every `every` steps; returns the objective J."""
+        J = ctypes.c_double()
+        self._check(self._lib.isph_adjoint_forward(self._h, ctypes.byref(obj), int(nsteps),
+                                                   int(every), ctypes.byref(J)), "adjoint_forward")
+        return J.value
+
+    def adjoint_backward(self):
+        """dJ/d(por, alpha, beta) of the last adjoint_forward, J, and the
+        BiCGSTAB iterations of the transposed solves."""
+        g = np.zeros(3)
+        J = ctypes.c_double()
+        it = ctypes.c_int64()
+        self._check(self._lib.isph_adjoint_backward(self._h, _dp(g), ctypes.byref(J),
+                                                    ctypes.byref(it)), "adjoint_backward")
+        return g, J.value, it.value
+
+    def adjoint_zero(self):
+        self._check(self._lib.isph_adjoint_zero(self._h), "adjoint_zero")
+
+    def adjoint_stage(self, stage, obj=None):
+        sid = ADJ_STAGE[stage] if isinstance(stage, str) else int(stage)
+        self._check(self._lib.isph_adjoint_stage(self._h, sid,
+                                                 ctypes.byref(obj) if obj is not None else None),
+                    "adjoint_stage")
+
+    def adjoint_get(self, name):
+        n = self.count()
+        w = 4 if name == "coff" else 1
+        out = np.empty(n * w, np.float64)
+        self._check(self._lib.isph_adjoint_get(self._h, FIELD[name], _dp(out)), "adjoint_get")
+        return out.reshape(n, 4) if w == 4 else out
+
+    def adjoint_set(self, name, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        self._check(self._lib.isph_adjoint_set(self._h, FIELD[name], _dp(v)), "adjoint_set")
+
+    def adjoint_params(self):
+        """(dJ/d(por, alpha, beta), dJ/d umax) accumulated by stage adjoints."""
+        g = np.zeros(3)
+        u = ctypes.c_double()
+        self._check(self._lib.isph_adjoint_params(self._h, _dp(g), ctypes.byref(u)),
+                    "adjoint_params")
+        return g, u.value
+
+    def objective_eval(self, obj):
+        J = ctypes.c_double()
+        self._check(self._lib.isph_objective_eval(self._h, ctypes.byref(obj), ctypes.byref(J)),
+                    "objective_eval")
+        return J.value
 
 
 def run_step_stages(sim, nsteps=1):

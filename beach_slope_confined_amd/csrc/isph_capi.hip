@@ -20,96 +20,15 @@
 
 using namespace isph;
 
-namespace {
+#include "isph_host.h"
 
-struct DevBuf {
-    StateView v;
-};
-
-struct EventPair {
-    hipEvent_t a, b;
-};
-
-struct Profiler {
-    bool on = false;
-    int stride = 1;        // record every stride-th launch of a slot
-    int64_t seen[4] = {0, 0, 0, 0};
-    bool cur[4] = {false, false, false, false};
-    std::vector<EventPair> pool[4];
-    std::vector<char> real[4];      // per recorded launch: did real work
-    size_t used[4] = {0, 0, 0, 0};
-    int64_t launches[4] = {0, 0, 0, 0};
-    double total_ms[4] = {0, 0, 0, 0};
-    int batch_recorded[4] = {0, 0, 0, 0};
-    int batch_issued[4] = {0, 0, 0, 0};
-};
-
-}  // namespace
-
-struct isph_ctx {
-    isph_params prm;
-    int32_t cap = 0, cap_soil = 0, n = 0, ns = 0;
-    GridInfo grid;
-    hipStream_t stream = nullptr;
-    StateView st[2];
-    int cur = 0;
-    double* L = nullptr;
-    double* divr = nullptr;
-    int *key = nullptr, *key_s = nullptr, *rank = nullptr, *perm = nullptr;
-    int *bincnt = nullptr, *scanbuf = nullptr;
-    int *nbr = nullptr, *cnt = nullptr;
-    bool have_nbr = false;
-    double *val64 = nullptr, *rhs = nullptr, *rsum = nullptr;
-    float* val32 = nullptr;
-    int* dpos = nullptr;
-    short* nbr16 = nullptr;
-    bool assembled = false;
-    bool assembled32 = false;
-    bool narrow_idx = false;
-    double* v64[12] = {};
-    float* v32[10] = {};
-    double2 *rq64 = nullptr, *rv64 = nullptr;
-    float2 *rq32 = nullptr, *rv32 = nullptr;
-    double* part = nullptr;
-    uint32_t* counters = nullptr;
-    SolverScalars* sc = nullptr;
-    SolverScalars* sc_host = nullptr;  // pinned
-    SimScalars* ss = nullptr;
-    uint32_t* err = nullptr;
-    double2* spos = nullptr;
-    double *smass = nullptr, *srho = nullptr;
-    int* sstart = nullptr;
-    int iters = 0, outer = 0;
-    double relres = 0.0;
-    Profiler prof;
-    struct Slot {
-        StateView v{};
-        int* key = nullptr;
-        SimScalars ss{};
-        int n = 0;
-        bool used = false;
-    } slots[64];
-    std::string msg;
-};
-
-namespace {
-
-#define HIPCHK(ctx, expr)                                                             \
-    do {                                                                              \
-        hipError_t e_ = (expr);                                                       \
-        if (e_ != hipSuccess) {                                                       \
-            if (ctx) (ctx)->msg = std::string(#expr) + ": " + hipGetErrorString(e_);  \
-            return ISPH_E_HIP;                                                        \
-        }                                                                             \
-    } while (0)
+namespace isph {
+namespace host {
 
 int fail(isph_ctx* c, int code, const char* m) {
     c->msg = m;
     return code;
 }
-
-inline int nblk(int64_t n, int b = BLOCK) { return (int)((n + b - 1) / b); }
-inline int nblk8(int64_t n) { return ((nblk(n) + 7) / 8) * 8; }
 
 GridInfo make_grid(const isph_params& p) {
     GridInfo g;
@@ -141,13 +60,6 @@ PorousParams porous_params(const isph_params& p) {
     pp.drag_law = p.drag_law;
     pp.visc_law = p.visc_law;
     return pp;
-}
-
-template <class T>
-int dalloc(isph_ctx* c, T** p, size_t count) {
-    HIPCHK(c, hipMalloc((void**)p, sizeof(T) * (count ? count : 1)));
-    HIPCHK(c, hipMemsetAsync(*p, 0, sizeof(T) * (count ? count : 1), c->stream));
-    return ISPH_OK;
 }
 
 int alloc_state(isph_ctx* c, StateView& s, size_t cap) {
@@ -439,15 +351,16 @@ SolveCtl solve_ctl(const isph_ctx* c, int mixed) {
     return t;
 }
 
-ResidArgs resid_args(isph_ctx* c, double* x, double* r) {
+ResidArgs resid_args(isph_ctx* c, double* x, double* r, const double* val = nullptr,
+                     const double* rhs = nullptr) {
     ResidArgs a;
     a.n = c->n;
     a.ld = (size_t)c->cap;
     a.nbr = c->nbr;
     a.cnt = c->cnt;
     a.dpos = c->dpos;
-    a.val = c->val64;
-    a.b = c->rhs;
+    a.val = val ? val : c->val64;
+    a.b = rhs ? rhs : c->rhs;
     a.x = x;
     a.r = r;
     a.part = c->part;
@@ -518,14 +431,13 @@ int iterate(isph_ctx* c, const KrylovArgs<TV, TX, TI>& a, const SolveCtl& ctl, b
     }
 }
 
-int solve_fp64(isph_ctx* c) {
+int solve_fp64_on(isph_ctx* c, const double* val, const double* rhs, double* x) {
     const int grid = nblk8(c->n);
-    double* x = c->v64[8];
     // the FP64 solve builds x in place
     double* vv[5] = {c->v64[0], c->v64[1], c->v64[2], c->v64[3], x};
-    auto ka = krylov_args<double, double, int>(c, c->nbr, c->val64, vv, c->rq64, c->rv64);
+    auto ka = krylov_args<double, double, int>(c, c->nbr, val, vv, c->rq64, c->rv64);
     const SolveCtl ctl = solve_ctl(c, 0);
-    ResidArgs ra = resid_args(c, x, c->v64[9]);
+    ResidArgs ra = resid_args(c, x, c->v64[9], val, rhs);
     k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 1, c->rq64, c->v64[0]);
     k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, ctl);
     HIPCHK(c, hipGetLastError());
@@ -551,6 +463,8 @@ int solve_fp64(isph_ctx* c) {
         if ((rc = iterate(c, ka, ctl, first, 0))) return rc;
     }
 }
+
+int solve_fp64(isph_ctx* c) { return solve_fp64_on(c, c->val64, c->rhs, c->v64[8]); }
 
 template <class TX, class TI>
 int solve_mixed_t(isph_ctx* c, const TI* nbr, TX* const* v, typename Pair2<TX>::type* rq,
@@ -592,8 +506,15 @@ int solve_mixed_t(isph_ctx* c, const TI* nbr, TX* const* v, typename Pair2<TX>::
         if (c->sc_host->done == DONE_CONVERGED) return ISPH_OK;
         if (c->sc_host->done == DONE_FAILED)
             return fail(c, ISPH_E_NOCONV, "mixed-precision PPE solve did not converge");
-        k_mx_replace<TX><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, rq, v[0], c->part, c->sc);
-        k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_REPLACE, grid, c->part, c->sc, ctl);
+        if (c->sc_host->brk) {
+            // breakdown: restart the recurrence from the true residual, rhat = r
+            k_mx_start<TX><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, rq, v[0], v[4], c->part, c->sc);
+            k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_START, grid, c->part, c->sc, ctl);
+            first = true;
+        } else {
+            k_mx_replace<TX><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, rq, v[0], c->part, c->sc);
+            k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_REPLACE, grid, c->part, c->sc, ctl);
+        }
         HIPCHK(c, hipGetLastError());
     }
 }
@@ -662,7 +583,26 @@ int run_step(isph_ctx* c) {
     return ISPH_OK;
 }
 
-}  // namespace
+int copy_state(isph_ctx* c, const StateView& a, const StateView& b, int n) {
+    auto D2D = hipMemcpyDeviceToDevice;
+    const size_t n2 = sizeof(double2) * n, nd = sizeof(double) * n, ni = sizeof(int) * n;
+    HIPCHK(c, hipMemcpyAsync(b.pos, a.pos, n2, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.vel, a.vel, n2, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.sp, a.sp, n2, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.sv, a.sv, n2, D2D, c->stream));
+    const double* da[] = {a.mass, a.rho, a.oden, a.p, a.por, a.R, a.nut, a.tke, a.varts, a.velm};
+    double* db[] = {b.mass, b.rho, b.oden, b.p, b.por, b.R, b.nut, b.tke, b.varts, b.velm};
+    for (int k = 0; k < 10; ++k) HIPCHK(c, hipMemcpyAsync(db[k], da[k], nd, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.tid, a.tid, ni, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.flags, a.flags, ni, D2D, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.pid, a.pid, ni, D2D, c->stream));
+    return ISPH_OK;
+}
+
+}  // namespace host
+}  // namespace isph
+
+using namespace isph::host;
 
 // ============================================================================
 // C ABI
@@ -754,6 +694,7 @@ int isph_destroy(isph_ctx* c) {
     for (int k = 0; k < 10; ++k)
         if (c->v32[k]) (void)hipFree(c->v32[k]);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
+    adj_free(c);
     for (auto& sl : c->slots) {
         if (!sl.v.pos) continue;
         free_state(sl.v);
@@ -937,6 +878,12 @@ static double* field_ptr(isph_ctx* c, int32_t f, int* width, bool* pair, int* co
         case ISPH_FIELD_VARTS: return s.varts;
         case ISPH_FIELD_VEL: return s.velm;
         case ISPH_FIELD_RHS: return c->rhs;
+        case ISPH_FIELD_X: *pair = true; *comp = 0; return (double*)s.pos;
+        case ISPH_FIELD_Y: *pair = true; *comp = 1; return (double*)s.pos;
+        case ISPH_FIELD_VX: *pair = true; *comp = 0; return (double*)s.vel;
+        case ISPH_FIELD_VY: *pair = true; *comp = 1; return (double*)s.vel;
+        case ISPH_FIELD_DENSITY: return s.rho;
+        case ISPH_FIELD_PRESSURE: return s.p;
         default: return nullptr;
     }
 }
@@ -1126,22 +1073,6 @@ int isph_step(isph_ctx* c, int32_t nsteps) {
     }
     prof_collect(c);
     return sync_check(c);
-}
-
-static int copy_state(isph_ctx* c, const StateView& a, const StateView& b, int n) {
-    auto D2D = hipMemcpyDeviceToDevice;
-    const size_t n2 = sizeof(double2) * n, nd = sizeof(double) * n, ni = sizeof(int) * n;
-    HIPCHK(c, hipMemcpyAsync(b.pos, a.pos, n2, D2D, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b.vel, a.vel, n2, D2D, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b.sp, a.sp, n2, D2D, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b.sv, a.sv, n2, D2D, c->stream));
-    const double* da[] = {a.mass, a.rho, a.oden, a.p, a.por, a.R, a.nut, a.tke, a.varts, a.velm};
-    double* db[] = {b.mass, b.rho, b.oden, b.p, b.por, b.R, b.nut, b.tke, b.varts, b.velm};
-    for (int k = 0; k < 10; ++k) HIPCHK(c, hipMemcpyAsync(db[k], da[k], nd, D2D, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b.tid, a.tid, ni, D2D, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b.flags, a.flags, ni, D2D, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b.pid, a.pid, ni, D2D, c->stream));
-    return ISPH_OK;
 }
 
 int isph_checkpoint(isph_ctx* c, int32_t slot) {

@@ -1,0 +1,144 @@
+// isph_host.h -- host-side context of libisph.so, shared by the translation
+// units of the library (forward step and C ABI: isph_capi.hip; discrete
+// adjoint: isph_adjoint.hip).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/isph_capi.h"
+#include "isph_kernels.h"
+
+namespace isph {
+namespace host {
+
+struct EventPair {
+    hipEvent_t a, b;
+};
+
+// HIP events around every stride-th launch of the four SpMV kernel slots
+struct Profiler {
+    bool on = false;
+    int stride = 1;        // record every stride-th launch of a slot
+    int64_t seen[4] = {0, 0, 0, 0};
+    bool cur[4] = {false, false, false, false};
+    std::vector<EventPair> pool[4];
+    std::vector<char> real[4];      // per recorded launch: did real work
+    size_t used[4] = {0, 0, 0, 0};
+    int64_t launches[4] = {0, 0, 0, 0};
+    double total_ms[4] = {0, 0, 0, 0};
+    int batch_recorded[4] = {0, 0, 0, 0};
+};
+
+struct AdjointState;  // isph_adjoint.hip
+
+}  // namespace host
+}  // namespace isph
+
+struct isph_ctx {
+    isph_params prm;
+    int32_t cap = 0, cap_soil = 0, n = 0, ns = 0;
+    isph::GridInfo grid;
+    hipStream_t stream = nullptr;
+    isph::StateView st[2];
+    int cur = 0;
+    double* L = nullptr;
+    double* divr = nullptr;
+    int *key = nullptr, *key_s = nullptr, *rank = nullptr, *perm = nullptr;
+    int *bincnt = nullptr, *scanbuf = nullptr;
+    int *nbr = nullptr, *cnt = nullptr;
+    bool have_nbr = false;
+    double *val64 = nullptr, *rhs = nullptr, *rsum = nullptr;
+    float* val32 = nullptr;
+    int* dpos = nullptr;
+    short* nbr16 = nullptr;
+    bool assembled = false;
+    bool assembled32 = false;
+    bool narrow_idx = false;
+    double* v64[12] = {};
+    float* v32[10] = {};
+    double2 *rq64 = nullptr, *rv64 = nullptr;
+    float2 *rq32 = nullptr, *rv32 = nullptr;
+    double* part = nullptr;
+    uint32_t* counters = nullptr;
+    isph::SolverScalars* sc = nullptr;
+    isph::SolverScalars* sc_host = nullptr;  // pinned
+    isph::SimScalars* ss = nullptr;
+    uint32_t* err = nullptr;
+    double2* spos = nullptr;
+    double *smass = nullptr, *srho = nullptr;
+    int* sstart = nullptr;
+    int iters = 0, outer = 0;
+    double relres = 0.0;
+    isph::host::Profiler prof;
+    struct Slot {
+        isph::StateView v{};
+        int* key = nullptr;
+        isph::SimScalars ss{};
+        int n = 0;
+        bool used = false;
+    } slots[64];
+    isph::host::AdjointState* adj = nullptr;
+    std::string msg;
+};
+
+#define HIPCHK(ctx, expr)                                                             \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            if (ctx) (ctx)->msg = std::string(#expr) + ": " + hipGetErrorString(e_);  \
+            return ISPH_E_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+namespace isph {
+namespace host {
+
+inline int nblk(int64_t n, int b = BLOCK) { return (int)((n + b - 1) / b); }
+inline int nblk8(int64_t n) { return ((nblk(n) + 7) / 8) * 8; }
+
+int fail(isph_ctx* c, int code, const char* m);
+PorousParams porous_params(const isph_params& p);
+KernelConsts kconsts(const isph_ctx* c);
+
+template <class T>
+int dalloc(isph_ctx* c, T** p, size_t count) {
+    HIPCHK(c, hipMalloc((void**)p, sizeof(T) * (count ? count : 1)));
+    HIPCHK(c, hipMemsetAsync(*p, 0, sizeof(T) * (count ? count : 1), c->stream));
+    return ISPH_OK;
+}
+int alloc_state(isph_ctx* c, StateView& s, size_t cap);
+void free_state(StateView& s);
+int copy_state(isph_ctx* c, const StateView& a, const StateView& b, int n);
+int take_errors(isph_ctx* c);
+int sync_check(isph_ctx* c);
+int read_sc(isph_ctx* c);
+void prof_collect(isph_ctx* c);
+
+// stages of the step, enqueue only (main.f90:41-66 order)
+int sort_particles(isph_ctx* c, bool with_sink);
+int run_projection(isph_ctx* c);
+int run_neighbour(isph_ctx* c);
+int run_effpor(isph_ctx* c);
+int run_eddyvis(isph_ctx* c);
+int run_int_vel(isph_ctx* c);
+int run_assemble(isph_ctx* c);
+int run_solve(isph_ctx* c);
+int run_comp_vel(isph_ctx* c);
+int run_comp_pos(isph_ctx* c);
+int run_freesurf(isph_ctx* c);
+int run_shift(isph_ctx* c);
+int run_timestep(isph_ctx* c);
+int run_step(isph_ctx* c);
+
+// FP64 BiCGSTAB on the current lists with the given off-diagonal values
+// (unit diagonal at dpos) and right-hand side; x holds the initial guess
+// on entry and the solution on return.
+int solve_fp64_on(isph_ctx* c, const double* val, const double* rhs, double* x);
+
+// discrete adjoint (isph_adjoint.hip)
+void adj_free(isph_ctx* c);
+
+}  // namespace host
+}  // namespace isph

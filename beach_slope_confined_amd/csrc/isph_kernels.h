@@ -51,8 +51,15 @@ struct SimScalars {
 
 struct SolverScalars {
     double bb, nb, prevr, prevr_prev, alpha, wi, betas, rr, rupd, nr, relres;
+    double rh2;  // ||rhat||^2 since the last (re)start
     int iters, total, first, done, status, nupd;
+    int brk;     // breakdown in this iteration: restart from the true residual
 };
+
+// Restart threshold on the shadow inner product rho = (r, rhat): its
+// rounding error is about eps ||r|| ||rhat||, so once |rho| <= BRK_COS
+// ||r|| ||rhat|| it has lost its digits and the recurrence restarts.
+constexpr double BRK_COS = 1e-14;
 
 // ---------------------------------------------------------------------------
 // fixed-tree reductions: wave64 butterfly (xor 1,2,..,32), then the four
@@ -117,6 +124,21 @@ struct KrylovArgs {
 };
 
 constexpr int FIN_BLOCK = 1024;  // threads of the one-block reduction kernel
+
+// cell (column-major, 1-based interior) of a position; out-of-grid
+// positions are clamped to the border cells and flagged
+__device__ inline int cell_of(const GridInfo& g, double x, double y, bool& oob) {
+    double fx = floor((x - g.xmin) / g.cs);
+    double fy = floor((y - g.ymin) / g.cs);
+    int cx, cy;
+    if (!(fx >= 0.0)) { cx = 1; oob = true; }
+    else if (fx > (double)(g.nx - 3)) { cx = g.nx - 2; oob = true; }
+    else cx = (int)fx + 1;
+    if (!(fy >= 0.0)) { cy = 1; oob = true; }
+    else if (fy > (double)(g.ny - 3)) { cy = g.ny - 2; oob = true; }
+    else cy = (int)fy + 1;
+    return cx * g.ny + cy;
+}
 
 // block sum over FIN_BLOCK threads: wave butterflies, then the 16 wave sums
 // as a balanced pairwise tree; valid in every thread

@@ -19,19 +19,6 @@ __device__ __forceinline__ void raise_err(uint32_t* err, uint32_t bit) { atomicO
 // (cellshift/cellshiftalt re-binning, integrator.f90:32-391, + resetid,
 // isph.f90:88-113)
 // ---------------------------------------------------------------------------
-__device__ inline int cell_of(const GridInfo& g, double x, double y, bool& oob) {
-    double fx = floor((x - g.xmin) / g.cs);
-    double fy = floor((y - g.ymin) / g.cs);
-    int cx, cy;
-    if (!(fx >= 0.0)) { cx = 1; oob = true; }
-    else if (fx > (double)(g.nx - 3)) { cx = g.nx - 2; oob = true; }
-    else cx = (int)fx + 1;
-    if (!(fy >= 0.0)) { cy = 1; oob = true; }
-    else if (fy > (double)(g.ny - 3)) { cy = g.ny - 2; oob = true; }
-    else cy = (int)fy + 1;
-    return cx * g.ny + cy;
-}
-
 __global__ void k_bin_keys(int n, GridInfo g, const double2* __restrict__ pos,
                            const int* __restrict__ tid, const int* __restrict__ flags, int sink,
                            double xlcut, double ytcut, int* __restrict__ key,

@@ -237,6 +237,8 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_finish(int kind, int nbp,
             sc->first = 1;
             sc->status = 0;
             sc->nupd = 0;
+            sc->rh2 = t1;
+            sc->brk = 0;
             if (t0 == 0.0) {
                 sc->done = DONE_ZERO_RHS;
                 sc->relres = 0.0;
@@ -248,12 +250,27 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_finish(int kind, int nbp,
             }
             break;
         }
-        case FIN_ALPHA:
-            sc->alpha = sc->prevr / t0;
+        // breakdown ((rhat, v) = 0, or (t, t) = 0 / (t, s) = 0): the step
+        // degenerates to a minimal-residual step (alpha = 0) or to the half
+        // step x + alpha p (omega = 0) and ends in a restart
+        case FIN_ALPHA: {
+            double a = sc->prevr / t0;
+            if (!isfinite(a)) {
+                a = 0.0;
+                sc->brk = 1;
+            }
+            sc->alpha = a;
             break;
-        case FIN_OMEGA:
-            sc->wi = t0 / t1;
+        }
+        case FIN_OMEGA: {
+            double w = t0 / t1;
+            if (!isfinite(w) || w == 0.0) {
+                w = 0.0;
+                sc->brk = 1;
+            }
+            sc->wi = w;
             break;
+        }
         case FIN_BETA: {
             const double betas = (t0 / sc->prevr) * (sc->alpha / sc->wi);
             sc->betas = betas;
@@ -264,7 +281,17 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_finish(int kind, int nbp,
             sc->total = sc->total + 1;
             sc->first = 0;
             const double nr = sqrt(t1);
-            if (!isfinite(betas) || !isfinite(t1)) {
+            if (!(fabs(t0) > BRK_COS * sqrt(t1 * sc->rh2))) sc->brk = 1;
+            if (sc->brk) {
+                // FP64: true residual + restart; mixed: flush + full restart
+                // (the flag stays set for the host until FIN_START)
+                if (!ctl.mixed) {
+                    sc->brk = 0;
+                    sc->done = DONE_RECURRENCE;
+                } else {
+                    sc->done = DONE_UPDATE;
+                }
+            } else if (!isfinite(betas) || !isfinite(t1)) {
                 sc->done = DONE_FAILED;
                 sc->status = ERR_NOCONV;
                 sc->relres = nr / sc->nb;
@@ -284,9 +311,14 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_finish(int kind, int nbp,
             if (nr <= ctl.tol * sc->nb) {
                 sc->done = DONE_CONVERGED;
                 sc->relres = nr / sc->nb;
+            } else if (!isfinite(nr)) {
+                sc->done = DONE_FAILED;
+                sc->status = ERR_NOCONV;
+                sc->relres = nr;
             } else {
                 sc->rr = t1;
                 sc->prevr = t1;
+                sc->rh2 = t1;
                 sc->first = 1;
                 sc->done = 0;
                 if (sc->total >= ctl.maxit) {
@@ -311,7 +343,8 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_finish(int kind, int nbp,
             sc->relres = sc->nr / sc->nb;
             if (sc->nr <= ctl.tol * sc->nb) {
                 sc->done = DONE_CONVERGED;
-            } else if (ctl.mixed == 2 && (sc->nupd >= ctl.maxupd || sc->total >= ctl.maxit)) {
+            } else if (ctl.mixed == 2 &&
+                       (sc->nupd >= ctl.maxupd || sc->total >= ctl.maxit || !isfinite(sc->nr))) {
                 sc->done = DONE_FAILED;
                 sc->status = ERR_NOCONV;
             } else {
@@ -322,6 +355,8 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_finish(int kind, int nbp,
         case FIN_START:
             sc->prevr = t0;
             sc->prevr_prev = t0;
+            sc->rh2 = t0;
+            sc->brk = 0;
             sc->rupd = sc->nr;
             sc->first = 1;
             sc->iters = 0;

@@ -26,6 +26,7 @@ FLAG_VICINITY = 8
 FIELD = {
     "xs": 0, "ys": 1, "vxs": 2, "vys": 3, "porosity": 4, "resist": 5,
     "coff": 6, "divr": 7, "nut": 8, "tke": 9, "varts": 10, "vel": 11, "rhs": 12,
+    "x": 13, "y": 14, "vx": 15, "vy": 16, "density": 17, "pressure": 18,
 }
 
 OK = 0

@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--nside", type=int, default=1024)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-dx", type=float, default=0.002)
+    ap.add_argument("--adjoint-steps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=400,
                     help="BiCGSTAB iterations timed on the CPU oracle")
     ap.add_argument("--pmc-run", action="store_true",
@@ -101,6 +104,45 @@ def run_gpu(cfg, steps, warmup, precision, profile):
         k0, k1 = (0, 1) if precision == PREC_FP64 else (2, 3)
         out["ktime"] = {"p_spmv": sim.kernel_time(k0), "s_spmv": sim.kernel_time(k1)}
     return out
+
+
+def run_adjoint(dx, nsteps, every):
+    """Adjoint/forward cost on the porous beach-slope dam break: nsteps FP64
+    steps with checkpoints every `every` steps, objective = time-mean
+    Shepard pressure at 3 probes on the slope face + water depth at
+    x = 1.2 m and x = 0.3 m; gradient with respect to (por, alpha, beta)."""
+    import torch
+    from beach_slope_confined_amd import adjoint, capi, configs
+    cfg = configs.dam_break(dx)
+    sim = configs.make_sim(cfg, capi.IsphSim)
+    sim.checkpoint(0)
+    obj = adjoint.slope_objective(gauges=(1.2, 0.3))
+    # warm-up: one short forward/backward (allocations, code paths)
+    sim.adjoint_forward(obj, 2, 1)
+    sim.adjoint_backward()
+    sim.restore(0)
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    sim.step(nsteps)  # plain forward, no checkpoints or objective
+    torch.cuda.synchronize()
+    plain = time.perf_counter() - a
+    sim.restore(0)
+    a = time.perf_counter()
+    J = sim.adjoint_forward(obj, nsteps, every)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - a
+    a = time.perf_counter()
+    g, _, adj_iters = sim.adjoint_backward()
+    torch.cuda.synchronize()
+    bwd = time.perf_counter() - a
+    n = len(cfg["parts"]["x"])
+    sim.close()
+    return {"workload": f"dam_break dx={dx:g}: {cfg['n_fluid']} fluid, {n} particles, "
+                        f"{len(cfg['soil']['x'])} soil; {nsteps} FP64 steps, checkpoints every {every}",
+            "forward_s": round(fwd, 4), "plain_forward_s": round(plain, 4),
+            "backward_s": round(bwd, 4), "adjoint_over_forward": round(bwd / fwd, 3),
+            "J": J, "grad_por_alpha_beta": [float(v) for v in g],
+            "adjoint_bicgstab_iters": int(adj_iters)}
 
 
 def cpu_baseline(cfg, gpu_iters):
@@ -185,10 +227,11 @@ def main():
     avg_s = tot / max(launches, 1) / 1e3
     nbytes = kernel_bytes(dom, r64["nnz_off"], r64["n"], 8, 8)
     achieved = nbytes / avg_s / 1e9 if launches else 0.0
+    kname = ("k_krylov_1t<double, double, int, false>" if dom == "p_spmv"
+             else "k_krylov_2<double, double, int>")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"k_krylov_{1 if dom == 'p_spmv' else 2}<double,double,int>",
-                "launches": launches, "avg_us": round(avg_s * 1e6, 2),
+                "kernel": kname, "launches": launches, "avg_us": round(avg_s * 1e6, 2),
                 "bytes_per_launch": nbytes}
     pmc = os.path.join(ROOT, "profiles", "bicg_spmv_traffic.json")
     if os.path.exists(pmc):
@@ -244,6 +287,12 @@ def main():
                                     "achieved_GBs": round(bm / avgm / 1e9, 1) if lm else None},
             }
             rmx["sim"].close()
+
+    if not args.no_adjoint:
+        try:
+            line["adjoint"] = run_adjoint(args.adjoint_dx, args.adjoint_steps, 10)
+        except Exception as e:  # reported, not hidden
+            line["adjoint"] = {"error": str(e)}
 
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, int(np.mean(timed_iters)))

@@ -130,6 +130,12 @@ typedef struct isph_ctx isph_ctx;
 #define ISPH_FIELD_VARTS   10   /* free-surface artificial viscosity (integrator.f90:495) */
 #define ISPH_FIELD_VEL     11   /* |v|/n (integrator.f90:825)                   */
 #define ISPH_FIELD_RHS     12   /* Jacobi-scaled PPE right-hand side            */
+#define ISPH_FIELD_X       13   /* position x (particle.f90:33)                 */
+#define ISPH_FIELD_Y       14   /* position y                                   */
+#define ISPH_FIELD_VX      15   /* velocity x (particle.f90:34)                 */
+#define ISPH_FIELD_VY      16   /* velocity y                                   */
+#define ISPH_FIELD_DENSITY 17   /* density rho = n * oden (porous.f90:190)       */
+#define ISPH_FIELD_PRESSURE 18  /* pressure (isph.f90:465-483)                  */
 
 /* ---- context ------------------------------------------------------------ */
 /* Allocates every device buffer for up to cap_particles particles and
@@ -225,6 +231,66 @@ int isph_get_kernel_time(isph_ctx* ctx, int32_t kernel, int64_t* launches,
  * `reps` times on the context stream; returns the mean ms per copy.  Used to
  * calibrate attainable HBM bandwidth and the FETCH_SIZE/WRITE_SIZE counters. */
 int isph_copy_bench(isph_ctx* ctx, int64_t bytes, int32_t reps, double* ms);
+
+/* ---- discrete adjoint (no reference counterpart: the reference has no
+ * derivatives; this is the calibration path for the porous parameters) ---- */
+/* Objective J = (1/nsteps) sum over steps of [ sum_k P_k + sum_g D_g ]
+ * evaluated after every step:
+ *   P_k  Shepard pressure at probe k, sum_j p_j m_j W_j / rho_j over
+ *        sum_j m_j W_j / rho_j, fluid j within 2h (the reference's probe,
+ *        probe.f90:86-130); 0 when no fluid particle is within 2h;
+ *   D_g  water depth at x = gauge_x[g]: sum_j (m_j / rho_j) w(x_j - x_g) over
+ *        fluid j with the normalised 1-D Wendland C2 weight
+ *        w(d) = 5/(8h) (1 - q/2)^3 (1.5 q + 1), q = |d|/h < 2.
+ * At most 8 probes and 8 gauges. */
+#define ISPH_OBJ_MAX 8
+typedef struct isph_objective {
+    int32_t nprobe, ngauge;
+    double probe_x[ISPH_OBJ_MAX], probe_y[ISPH_OBJ_MAX];
+    double gauge_x[ISPH_OBJ_MAX];
+} isph_objective;
+
+/* Forward run for the adjoint: porous closure + timestep at the current
+ * state (as before the reference's time loop, main.f90:35-37), then nsteps
+ * steps; the state at the start of every `ckpt_every`-th step and the final
+ * state are kept in device memory.  Returns J.  Needs a fixed dt
+ * (adaptive_dt = 0) and no outflow sink. */
+int isph_adjoint_forward(isph_ctx* ctx, const isph_objective* obj, int32_t nsteps,
+                         int32_t ckpt_every, double* J);
+/* Reverse sweep of the last isph_adjoint_forward (same parameters): for each
+ * checkpoint segment, last first, recomputes its steps with a per-step tape
+ * and runs the adjoint of every stage in reverse order; the PPE adjoint is
+ * one BiCGSTAB solve with the transposed matrix per step.  Returns
+ * grad = dJ/d(por, alpha, beta), J, and the total BiCGSTAB iterations of the
+ * adjoint solves.  Leaves the context at the final state of the forward run. */
+int isph_adjoint_backward(isph_ctx* ctx, double* grad3, double* J, int64_t* adj_iters);
+
+/* Stage adjoints on the current state, for testing the reverse sweep one
+ * stage at a time.  The adjoint arrays (same field ids as isph_get_field;
+ * ISPH_FIELD_COFF is the 4-per-particle L) hold the adjoint of the stage's
+ * outputs on entry and are updated in place to the adjoint of its inputs.
+ * The state must be the stage's input state (for ISPH_ADJ_PPESOLVE: after
+ * the forward solve, whose pressure it uses); list stages use the lists and
+ * L of the last isph_neighbour. */
+#define ISPH_ADJ_PROJECTION 1
+#define ISPH_ADJ_NEIGHBOUR  3   /* KGC matrices L = inv(A)             */
+#define ISPH_ADJ_EFFPOR     4
+#define ISPH_ADJ_EDDYVIS    5
+#define ISPH_ADJ_VARTS      6
+#define ISPH_ADJ_VSTAR      7
+#define ISPH_ADJ_PPESOLVE   8   /* assembly + transposed solve         */
+#define ISPH_ADJ_COMP_VEL  10
+#define ISPH_ADJ_COMP_POS  11
+#define ISPH_ADJ_SHIFT     16   /* opt2_shift + its position/velocity update */
+#define ISPH_ADJ_OBJECTIVE 20   /* one step's objective term, weight 1 */
+int isph_adjoint_stage(isph_ctx* ctx, int32_t stage, const isph_objective* obj);
+int isph_adjoint_zero(isph_ctx* ctx);
+int isph_adjoint_get(isph_ctx* ctx, int32_t field, double* out);
+int isph_adjoint_set(isph_ctx* ctx, int32_t field, const double* in);
+/* dJ/d(por, alpha, beta) accumulated by the stage adjoints, and dJ/d umax */
+int isph_adjoint_params(isph_ctx* ctx, double* grad3, double* umax_bar);
+/* one step's objective term at the current state */
+int isph_objective_eval(isph_ctx* ctx, const isph_objective* obj, double* J);
 
 #ifdef __cplusplus
 }

@@ -982,6 +982,12 @@ static void spmv_csr_f(int32_t n, const int32_t* rp, const int32_t* col, const f
     }
 }
 
+/* Restart threshold on the BiCGSTAB shadow inner product: rho = (r, rhat)
+ * carries a rounding error of about eps ||r|| ||rhat||, so once
+ * |rho| <= BRK_COS ||r|| ||rhat|| it has lost its digits and the recurrence
+ * restarts from the true residual with rhat = r. */
+#define BRK_COS 1e-14
+
 /* BiCGSTAB with the recurrences of solver.f90:162-213 (van der Vorst 1992),
  * stop when ||r||_2 <= tol ||b||_2 on the recurrence residual and then on the
  * true residual b - Ax; a failed true-residual check restarts from it. */
@@ -1005,6 +1011,7 @@ static int bicgstab_core(int32_t n, const int32_t* rp, const int32_t* col, const
     }
     double rr = ora_dot_tree(rs, rs, n);
     double prevr = rr;
+    double rh2 = rr; /* ||rhat||^2 */
     if (sqrt(rr) <= tol * nb) { *relres = sqrt(rr) / nb; return ISPH_OK; }
     int it = 0;
     for (;;) {
@@ -1013,11 +1020,18 @@ static int bicgstab_core(int32_t n, const int32_t* rp, const int32_t* col, const
         spmv_csr(n, rp, col, val, p, mm1);
         double d1 = ora_dot_tree(mm1, res, n);
         double alpha = prevr / d1;
+        /* breakdown ((rhat, v) = 0, or (t, t) = 0 / (t, s) = 0 below): the
+         * step degenerates to a minimal-residual step (alpha = 0) or to the
+         * half step x + alpha p (omega = 0), then restarts from the true
+         * residual with rhat = r */
+        int brk = 0;
+        if (!isfinite(alpha)) { alpha = 0.0; brk = 1; }
         for (int32_t i = 0; i < n; ++i) si[i] = rs[i] - alpha * mm1[i];
         spmv_csr(n, rp, col, val, si, mm2);
         double d2 = ora_dot_tree(mm2, si, n);
         double d3 = ora_dot_tree(mm2, mm2, n);
         double wi = d2 / d3;
+        if (!isfinite(wi) || wi == 0.0) { wi = 0.0; brk = 1; }
         for (int32_t i = 0; i < n; ++i) {
             sl[i] = (sl[i] + alpha * p[i]) + wi * si[i];
             rs[i] = si[i] - wi * mm2[i];
@@ -1026,19 +1040,22 @@ static int bicgstab_core(int32_t n, const int32_t* rp, const int32_t* col, const
         rr = ora_dot_tree(rs, rs, n);
         double betas = (d4 / prevr) * (alpha / wi);
         prevr = d4;
-        if (!isfinite(betas) || !isfinite(rr)) {
+        if (!(fabs(d4) > BRK_COS * sqrt(rr * rh2))) brk = 1;
+        if (!brk && (!isfinite(betas) || !isfinite(rr))) {
             *iters = it; *relres = NAN;
             return ISPH_E_NOCONV;
         }
-        if (sqrt(rr) <= tol * nb) {
+        if (brk || sqrt(rr) <= tol * nb) {
             /* true residual */
             spmv_csr(n, rp, col, val, sl, mm1);
             for (int32_t i = 0; i < n; ++i) rs[i] = b[i] - mm1[i];
             double tr = ora_dot_tree(rs, rs, n);
             if (sqrt(tr) <= tol * nb) { *iters = it; *relres = sqrt(tr) / nb; return ISPH_OK; }
+            if (!isfinite(tr)) { *iters = it; *relres = NAN; return ISPH_E_NOCONV; }
             for (int32_t i = 0; i < n; ++i) { p[i] = rs[i]; res[i] = rs[i]; }
             rr = tr;
             prevr = tr;
+            rh2 = tr;
             continue;
         }
         for (int32_t i = 0; i < n; ++i) p[i] = rs[i] + betas * (p[i] - wi * mm1[i]);
@@ -1113,7 +1130,7 @@ static int NAME(ora_ctx* c, const float* val32, double* x, int32_t* iters, int32
         res[i] = rs[i];                                                                     \
         sl[i] = (TX)0;                                                                      \
     }                                                                                       \
-    double prevr = DOT(res, rs, n), prevr_prev = prevr;                                     \
+    double prevr = DOT(res, rs, n), prevr_prev = prevr, rh2 = prevr;                        \
     double alpha = 0.0, wi = 0.0, betas = 0.0;                                              \
     int first = 1;                                                                          \
     int32_t total = 0, nupd = 0;                                                            \
@@ -1130,12 +1147,15 @@ static int NAME(ora_ctx* c, const float* val32, double* x, int32_t* iters, int32
         SPMV(n, c->rowptr, c->col, val32, p, mm1);                                          \
         double d1 = DOT(mm1, res, n);                                                       \
         alpha = prevr / d1;                                                                 \
+        int brk = 0;                                                                        \
+        if (!isfinite(alpha)) { alpha = 0.0; brk = 1; }                                     \
         TX at = (TX)alpha;                                                                  \
         for (int32_t i = 0; i < n; ++i) si[i] = rs[i] - at * mm1[i];                        \
         SPMV(n, c->rowptr, c->col, val32, si, mm2);                                         \
         double d2 = DOT(mm2, si, n);                                                        \
         double d3 = DOT(mm2, mm2, n);                                                       \
         wi = d2 / d3;                                                                       \
+        if (!isfinite(wi) || wi == 0.0) { wi = 0.0; brk = 1; }                              \
         TX wt = (TX)wi;                                                                     \
         for (int32_t i = 0; i < n; ++i) {                                                   \
             sl[i] = (sl[i] + at * p[i]) + wt * si[i];                                       \
@@ -1146,8 +1166,9 @@ static int NAME(ora_ctx* c, const float* val32, double* x, int32_t* iters, int32
         betas = (d4 / prevr) * (alpha / wi);                                                \
         prevr_prev = prevr;                                                                 \
         prevr = d4;                                                                         \
-        if (!isfinite(betas) || !isfinite(rrk)) { rc = ISPH_E_NOCONV; break; }              \
-        if (sqrt(rrk) <= prm->tol * nb || sqrt(rrk) <= prm->inner_tol * rupd) {             \
+        if (!(fabs(d4) > BRK_COS * sqrt(rrk * rh2))) brk = 1;                               \
+        if (!brk && (!isfinite(betas) || !isfinite(rrk))) { rc = ISPH_E_NOCONV; break; }    \
+        if (brk || sqrt(rrk) <= prm->tol * nb || sqrt(rrk) <= prm->inner_tol * rupd) {      \
             for (int32_t i = 0; i < n; ++i) {                                               \
                 x[i] = x[i] + (double)sl[i];                                                \
                 sl[i] = (TX)0;                                                              \
@@ -1158,12 +1179,23 @@ static int NAME(ora_ctx* c, const float* val32, double* x, int32_t* iters, int32
             nupd++;                                                                         \
             *relres = sqrt(tr) / nb;                                                        \
             if (sqrt(tr) <= prm->tol * nb) break;                                           \
-            if (nupd >= prm->max_outer || total >= prm->maxit) { rc = ISPH_E_NOCONV; break; } \
+            if (nupd >= prm->max_outer || total >= prm->maxit || !isfinite(tr)) {          \
+                rc = ISPH_E_NOCONV;                                                         \
+                break;                                                                      \
+            }                                                                               \
             for (int32_t i = 0; i < n; ++i) rs[i] = (TX)r64[i];                             \
             rupd = sqrt(tr);                                                                \
-            d4 = DOT(rs, res, n);                                                           \
-            betas = (d4 / prevr_prev) * (alpha / wi);                                       \
-            prevr = d4;                                                                     \
+            if (brk) { /* full restart: rhat = r */                                         \
+                for (int32_t i = 0; i < n; ++i) res[i] = rs[i];                             \
+                prevr = DOT(res, rs, n);                                                    \
+                prevr_prev = prevr;                                                         \
+                rh2 = prevr;                                                                \
+                first = 1;                                                                  \
+            } else {                                                                        \
+                d4 = DOT(rs, res, n);                                                       \
+                betas = (d4 / prevr_prev) * (alpha / wi);                                   \
+                prevr = d4;                                                                 \
+            }                                                                               \
         }                                                                                   \
     }                                                                                       \
     *iters = total;                                                                         \
