@@ -59,7 +59,7 @@ struct SolverScalars {
 // Restart threshold on the shadow inner product rho = (r, rhat): its
 // rounding error is about eps ||r|| ||rhat||, so once |rho| <= BRK_COS
 // ||r|| ||rhat|| it has lost its digits and the recurrence restarts.
-constexpr double BRK_COS = 1e-14;
+constexpr double BRK_COS = 1e-15;
 
 // ---------------------------------------------------------------------------
 // fixed-tree reductions: wave64 butterfly (xor 1,2,..,32), then the four

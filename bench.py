@@ -61,8 +61,8 @@ def kernel_bytes(kind, nnz_off, n, val_bytes, vec_bytes):
     gathered vector counted once per row, own-row reads and writes once."""
     idx = 4 * nnz_off + 8 * n            # nbr + cnt + dpos
     val = val_bytes * nnz_off
-    if kind == "p_spmv":                 # gathers r, q; reads rhat; writes p, v
-        vec = vec_bytes * (2 + 1 + 2) * n
+    if kind == "p_spmv":                 # gathers (r, q); reads rhat; writes p and (r, v)
+        vec = vec_bytes * (2 + 1 + 1 + 2) * n
     else:                                # gathers r, v; writes s, t
         vec = vec_bytes * (2 + 2) * n
     return idx + val + vec

@@ -986,7 +986,7 @@ static void spmv_csr_f(int32_t n, const int32_t* rp, const int32_t* col, const f
  * carries a rounding error of about eps ||r|| ||rhat||, so once
  * |rho| <= BRK_COS ||r|| ||rhat|| it has lost its digits and the recurrence
  * restarts from the true residual with rhat = r. */
-#define BRK_COS 1e-14
+#define BRK_COS 1e-15
 
 /* BiCGSTAB with the recurrences of solver.f90:162-213 (van der Vorst 1992),
  * stop when ||r||_2 <= tol ||b||_2 on the recurrence residual and then on the
