@@ -431,11 +431,13 @@ int iterate(isph_ctx* c, const KrylovArgs<TV, TX, TI>& a, const SolveCtl& ctl, b
     }
 }
 
-int solve_fp64_on(isph_ctx* c, const double* val, const double* rhs, double* x) {
+template <class TI>
+int solve_fp64_t(isph_ctx* c, const TI* nbr, const double* val, const double* rhs,
+                 double* x) {
     const int grid = nblk8(c->n);
     // the FP64 solve builds x in place
     double* vv[5] = {c->v64[0], c->v64[1], c->v64[2], c->v64[3], x};
-    auto ka = krylov_args<double, double, int>(c, c->nbr, val, vv, c->rq64, c->rv64);
+    auto ka = krylov_args<double, double, TI>(c, nbr, val, vv, c->rq64, c->rv64);
     const SolveCtl ctl = solve_ctl(c, 0);
     ResidArgs ra = resid_args(c, x, c->v64[9], val, rhs);
     k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 1, c->rq64, c->v64[0]);
@@ -462,6 +464,24 @@ int solve_fp64_on(isph_ctx* c, const double* val, const double* rhs, double* x) 
         }
         if ((rc = iterate(c, ka, ctl, first, 0))) return rc;
     }
+}
+
+// 16-bit column offsets j - i when every offset of the lists fits (the index
+// stream of the two SpMVs per iteration drops from 4 to 2 bytes per entry),
+// else the int32 columns; the arithmetic is the same either way.
+int solve_fp64_on(isph_ctx* c, const double* val, const double* rhs, double* x) {
+    k_to_short<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, (size_t)c->cap, c->nbr, c->cnt, c->nbr16,
+                                                     c->err);
+    uint32_t e = 0;
+    HIPCHK(c, hipMemcpyAsync(&e, c->err, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (e & ERR_INDEX16) {
+        HIPCHK(c, hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->stream));
+        c->narrow_idx = false;
+        return solve_fp64_t<int>(c, c->nbr, val, rhs, x);
+    }
+    c->narrow_idx = true;
+    return solve_fp64_t<short>(c, c->nbr16, val, rhs, x);
 }
 
 int solve_fp64(isph_ctx* c) { return solve_fp64_on(c, c->val64, c->rhs, c->v64[8]); }

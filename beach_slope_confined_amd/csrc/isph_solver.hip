@@ -5,7 +5,7 @@
 // one-block reduction kernel (k_finish) that turns the block partials into
 // the iteration's scalars and stop flags:
 //   k_krylov_1  p = r + beta q (or p = r after a (re)start), v = A p, (v, rhat)
-//   k_krylov_2  s = r - alpha v, t = A s, (t, s), (t, t)
+//   k_krylov_2  s = r - alpha v, t = A s, (t, s), (t, t)   (rows in reverse order)
 //   k_krylov_3  x += alpha p + omega s, r = s - omega t, q = p - omega v,
 //               (r, rhat), (r, r)
 // q carries p - omega v from one iteration to the next so that the p update
@@ -15,7 +15,7 @@
 // makes them visible, so no fences or global atomics are needed.
 //
 // Flavours (template TV matrix value, TX vector, TI column index):
-//   <double, double, int>    FP64 drop-in
+//   <double, double, short>  FP64 drop-in (int when an offset does not fit 16 bits)
 //   <float,  float,  short>  mixed: FP32 values and FP32 Krylov vectors
 //   <float,  double, short>  mixed: FP32 values, FP64 Krylov vectors,
 //                            conservation-form SpMV with FP64 row sums
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(BLOCK) k_krylov_2(KrylovArgs<TV, TX, TI> a,
                                                     const SolverScalars* sc) {
     using P2 = typename Pair2<TX>::type;
     if (sc->done) return;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int lb = xcd_block_rev(blockIdx.x, gridDim.x);  // opposite order to k_krylov_1t
     const int i = lb * BLOCK + threadIdx.x;
     const TX alpha = (TX)sc->alpha;
     const P2* __restrict__ rv = a.rv;
@@ -476,6 +476,14 @@ template __global__ void k_krylov_2<double, double, int>(KrylovArgs<double, doub
                                                          const SolverScalars*);
 template __global__ void k_krylov_3<double, double, int>(KrylovArgs<double, double, int>,
                                                          const SolverScalars*);
+template __global__ void k_krylov_1t<double, double, short, true>(KrylovArgs<double, double, short>,
+                                                                 const SolverScalars*);
+template __global__ void k_krylov_1t<double, double, short, false>(KrylovArgs<double, double, short>,
+                                                                  const SolverScalars*);
+template __global__ void k_krylov_2<double, double, short>(KrylovArgs<double, double, short>,
+                                                           const SolverScalars*);
+template __global__ void k_krylov_3<double, double, short>(KrylovArgs<double, double, short>,
+                                                           const SolverScalars*);
 #define ISPH_MIXED_FLAVOUR(TX, TI)                                                              \
     template __global__ void k_krylov_1t<float, TX, TI, true>(KrylovArgs<float, TX, TI>,        \
                                                               const SolverScalars*);           \

@@ -55,11 +55,12 @@ def parse():
     return ap.parse_args()
 
 
-def kernel_bytes(kind, nnz_off, n, val_bytes, vec_bytes):
+def kernel_bytes(kind, nnz_off, n, val_bytes, vec_bytes, idx_bytes=4):
     """Algorithmic bytes of one launch of a BiCGSTAB SpMV kernel: every
-    neighbour index and matrix value once, per row cnt and dpos, each
-    gathered vector counted once per row, own-row reads and writes once."""
-    idx = 4 * nnz_off + 8 * n            # nbr + cnt + dpos
+    neighbour index (4-byte column or 2-byte offset) and matrix value once,
+    per row cnt and dpos, each gathered vector counted once per row, own-row
+    reads and writes once."""
+    idx = idx_bytes * nnz_off + 8 * n    # nbr + cnt + dpos
     val = val_bytes * nnz_off
     if kind == "p_spmv":                 # gathers (r, q); reads rhat; writes p and (r, v)
         vec = vec_bytes * (2 + 1 + 1 + 2) * n
@@ -97,9 +98,12 @@ def run_gpu(cfg, steps, warmup, precision, profile):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     s = sim.scalars()
-    ptr, _ = sim.get_neighbours()
+    ptr, idx = sim.get_neighbours()
+    rows = np.repeat(np.arange(len(ptr) - 1), np.diff(ptr))
+    # the solver reads 16-bit column offsets when every offset fits
+    narrow = bool(len(idx) == 0 or np.abs(idx.astype(np.int64) - rows).max() <= 32767)
     out = dict(sim=sim, seconds=t1 - t0, iters=iters, relres=s["relres"], outer=s["outer"],
-               nnz_off=int(ptr[-1]), n=len(ptr) - 1, first=first)
+               nnz_off=int(ptr[-1]), n=len(ptr) - 1, first=first, idx_bytes=2 if narrow else 4)
     if profile:
         k0, k1 = (0, 1) if precision == PREC_FP64 else (2, 3)
         out["ktime"] = {"p_spmv": sim.kernel_time(k0), "s_spmv": sim.kernel_time(k1)}
@@ -225,10 +229,11 @@ def main():
     dom = max(kt, key=lambda k: kt[k][1])
     launches, tot = kt[dom]
     avg_s = tot / max(launches, 1) / 1e3
-    nbytes = kernel_bytes(dom, r64["nnz_off"], r64["n"], 8, 8)
+    nbytes = kernel_bytes(dom, r64["nnz_off"], r64["n"], 8, 8, r64["idx_bytes"])
     achieved = nbytes / avg_s / 1e9 if launches else 0.0
-    kname = ("k_krylov_1t<double, double, int, false>" if dom == "p_spmv"
-             else "k_krylov_2<double, double, int>")
+    ti = "short" if r64["idx_bytes"] == 2 else "int"
+    kname = (f"k_krylov_1t<double, double, {ti}, false>" if dom == "p_spmv"
+             else f"k_krylov_2<double, double, {ti}>")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": kname, "launches": launches, "avg_us": round(avg_s * 1e6, 2),
@@ -273,7 +278,7 @@ def main():
             lm, tm = ktm[domm]
             avgm = tm / max(lm, 1) / 1e3
             # FP32 values, 16-bit column offsets, vectors of vb bytes
-            bm = kernel_bytes(domm, rmx["nnz_off"], rmx["n"], 4, vb) - 2 * rmx["nnz_off"]
+            bm = kernel_bytes(domm, rmx["nnz_off"], rmx["n"], 4, vb, rmx["idx_bytes"])
             line[key] = {
                 "value": round(vmx, 1), "ms_per_step": round(1e3 * rmx["seconds"] / args.steps, 3),
                 "speedup_vs_fp64": round(vmx / value, 3),
@@ -281,8 +286,9 @@ def main():
                 "iters_per_step": rmx["iters"][args.warmup:], "reliable_updates": rmx["outer"],
                 "relres": rmx["relres"], "pressure_err_vs_fp64": perr,
                 "velocity_err_vs_fp64": verr,
-                "dominant_kernel": {"kernel": f"k_krylov_{1 if domm == 'p_spmv' else 2}<float,"
-                                              f"{'double' if vb == 8 else 'float'},short>",
+                "dominant_kernel": {"kernel": f"k_krylov_{'1t' if domm == 'p_spmv' else 2}<float, "
+                                              f"{'double' if vb == 8 else 'float'}, "
+                                              f"{'short' if rmx['idx_bytes'] == 2 else 'int'}>",
                                     "avg_us": round(avgm * 1e6, 2),
                                     "achieved_GBs": round(bm / avgm / 1e9, 1) if lm else None},
             }
