@@ -28,7 +28,7 @@ _c_ip = ctypes.POINTER(ctypes.c_int32)
 SUBROUTINES = [
     "projection", "cellshift", "resetid", "neighbour", "effpor", "eddyvis",
     "int_vel", "ppe_assemble", "comp_vel", "comp_pos", "cellshiftalt",
-    "freesurf", "opt2_shift", "timestep",
+    "freesurf", "opt2_shift", "timestep", "scalart", "densityupdate", "eddyvis_lagged",
 ]
 
 
@@ -96,6 +96,9 @@ def declare(lib, prefix):
         "ppe_solve": [P, _c_ip, _c_dp],
         "ppesolve": [P, _c_ip, _c_dp],
         "step": [P, i32],
+        "scalarupdate": [P, ctypes.c_double],
+        "salt_init": [P, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+        "get_substeps": [P, _c_ip, _c_dp],
     }
     for name in SUBROUTINES:
         sig[name] = [P]
@@ -318,6 +321,26 @@ class Sim:
 
     def step(self, nsteps=1):
         self._check(self._fn("step")(self._h, int(nsteps)), "step")
+
+    # -- salinity transport (scalar.f90) -------------------------------------
+    def scalart(self): return self.call("scalart")
+    def eddyvis_lagged(self): return self.call("eddyvis_lagged")
+    def densityupdate(self): return self.call("densityupdate")
+
+    def scalarupdate(self, stime):
+        self._check(self._fn("scalarupdate")(self._h, float(stime)), "scalarupdate")
+
+    def salt_init(self, xl, yl, line_grad):
+        """Salt-water start of main.f90:160-195."""
+        self._check(self._fn("salt_init")(self._h, float(xl), float(yl), float(line_grad)),
+                    "salt_init")
+
+    def substeps(self):
+        """(solsteps, dtsol) of the next step's transport."""
+        n, d = ctypes.c_int32(), ctypes.c_double()
+        self._check(self._fn("get_substeps")(self._h, ctypes.byref(n), ctypes.byref(d)),
+                    "get_substeps")
+        return n.value, d.value
 
     def init_state(self):
         """Start-up as main.f90:35-37 (effpor) plus one timestep() so that

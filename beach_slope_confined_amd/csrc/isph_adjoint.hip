@@ -472,6 +472,8 @@ int reverse_step(isph_ctx* c, StepTape& t, const StateView& end, double weight, 
 int check_adjoint_params(isph_ctx* c) {
     if (c->prm.adaptive_dt) return fail(c, ISPH_E_ARG, "the adjoint needs a fixed dt (adaptive_dt = 0)");
     if (c->prm.sink) return fail(c, ISPH_E_ARG, "the adjoint does not support the outflow sink");
+    if (c->prm.salinity)
+        return fail(c, ISPH_E_ARG, "the adjoint does not support salinity transport (salinity = 0)");
     return ISPH_OK;
 }
 

@@ -68,7 +68,8 @@ int alloc_state(isph_ctx* c, StateView& s, size_t cap) {
     rc |= dalloc(c, &s.vel, cap);
     rc |= dalloc(c, &s.sp, cap);
     rc |= dalloc(c, &s.sv, cap);
-    double** d[] = {&s.mass, &s.rho, &s.oden, &s.p, &s.por, &s.R, &s.nut, &s.tke, &s.varts, &s.velm};
+    double** d[] = {&s.mass, &s.rho, &s.oden, &s.p, &s.por, &s.R, &s.nut, &s.tke, &s.varts, &s.velm,
+                    &s.con, &s.ovol};
     for (auto pp : d) rc |= dalloc(c, pp, cap);
     rc |= dalloc(c, &s.tid, cap);
     rc |= dalloc(c, &s.flags, cap);
@@ -78,7 +79,7 @@ int alloc_state(isph_ctx* c, StateView& s, size_t cap) {
 
 void free_state(StateView& s) {
     void* p[] = {s.pos, s.vel, s.sp, s.sv, s.mass, s.rho, s.oden, s.p, s.por, s.R, s.nut, s.tke,
-                 s.varts, s.velm, s.tid, s.flags, s.pid};
+                 s.varts, s.velm, s.con, s.ovol, s.tid, s.flags, s.pid};
     for (auto q : p)
         if (q) (void)hipFree(q);
 }
@@ -95,6 +96,8 @@ int take_errors(isph_ctx* c) {
     if (e & ERR_CAPACITY) return fail(c, ISPH_E_CAPACITY, "neighbour list capacity exceeded");
     if (e & ERR_DOMAIN) return fail(c, ISPH_E_DOMAIN, "particle outside the cell grid");
     if (e & ERR_NOCONV) return fail(c, ISPH_E_NOCONV, "PPE solve did not converge");
+    if (e & ERR_SUBSTEPS)
+        return fail(c, ISPH_E_CAPACITY, "salinity transport needs more than 2^20 substeps");
     return fail(c, ISPH_E_HIP, "unknown device error");
 }
 
@@ -167,6 +170,13 @@ int run_neighbour(isph_ctx* c) {
             c->n, c->grid, kconsts(c), p.lamfs * p.maxdivr, p.max_nbr, (size_t)c->cap,
             c->st[c->cur], c->key_s, c->bincnt, c->nbr, c->cnt, c->L, c->divr, c->err);
     HIPCHK(c, hipGetLastError());
+    // the lists keep these positions' distances (the reference stores dist and
+    // klt per pair, search.f90:65-74): the stages of the salt-water step that
+    // run after the shift has moved the particles (scalart, the last eddyvis)
+    // take their pair distances from here
+    if (c->n)
+        HIPCHK(c, hipMemcpyAsync(c->pos_nb, c->st[c->cur].pos, sizeof(double2) * c->n,
+                                 hipMemcpyDeviceToDevice, c->stream));
     c->have_nbr = true;
     c->assembled = false;
     return ISPH_OK;
@@ -182,14 +192,24 @@ int run_effpor(isph_ctx* c) {
     return ISPH_OK;
 }
 
-int run_eddyvis(isph_ctx* c) {
+// Pair geometry from the current positions (the list stages of this library
+// take it from there; in the reference's step order that is where the last
+// search left the particles) or, with `stored`, from the positions of the
+// last search, as the reference's stored klt (turbulence.f90:36) for the
+// eddyvis that main.f90:260 calls after the shift has moved them.
+static int eddyvis_on(isph_ctx* c, bool stored) {
     if (!c->have_nbr) return fail(c, ISPH_E_STATE, "eddyvis before neighbour");
+    StateView v = c->st[c->cur];
+    if (stored) v.pos = c->pos_nb;
     if (c->n)
         k_eddyvis<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, kconsts(c), c->prm.dl, (size_t)c->cap,
-                                                        c->st[c->cur], c->nbr, c->cnt, c->L);
+                                                        v, c->nbr, c->cnt, c->L);
     HIPCHK(c, hipGetLastError());
     return ISPH_OK;
 }
+
+int run_eddyvis(isph_ctx* c) { return eddyvis_on(c, false); }
+int run_eddyvis_lagged(isph_ctx* c) { return eddyvis_on(c, true); }
 
 int run_int_vel(isph_ctx* c) {
     if (!c->have_nbr) return fail(c, ISPH_E_STATE, "int_vel before neighbour");
@@ -266,7 +286,7 @@ int run_timestep(isph_ctx* c) {
         k_fill<<<1, BLOCK, 0, c->stream>>>(2, c->part, 0.0);
     const isph_params& p = c->prm;
     k_timestep<<<1, BLOCK, 0, c->stream>>>(nb, c->part, c->ss, p.dl, p.sig1, p.mu, p.rho0,
-                                           p.adaptive_dt);
+                                           p.adaptive_dt, p.tschmidt, p.salinity, c->err);
     HIPCHK(c, hipGetLastError());
     return ISPH_OK;
 }
@@ -580,9 +600,13 @@ int run_solve(isph_ctx* c) {
     return ISPH_OK;
 }
 
+// One step: main.f90:41-66, or with salinity main.f90:199-263 (transport
+// substeps first, densityupdate after the shift, eddyvis after timestep).
 int run_step(isph_ctx* c) {
     int rc;
+    const bool salt = c->prm.salinity != 0;
     k_advance_time<<<1, 64, 0, c->stream>>>(c->ss);
+    if (salt && (rc = run_salt_substeps(c))) return rc;
     if ((rc = run_projection(c))) return rc;
     if ((rc = sort_particles(c, false))) return rc;
     if ((rc = run_neighbour(c))) return rc;
@@ -599,7 +623,9 @@ int run_step(isph_ctx* c) {
     if ((rc = run_freesurf(c))) return rc;
     if ((rc = run_effpor(c))) return rc;
     if ((rc = run_shift(c))) return rc;
+    if (salt && (rc = run_densityupdate(c))) return rc;
     if ((rc = run_timestep(c))) return rc;
+    if (salt && (rc = run_eddyvis_lagged(c))) return rc;
     return ISPH_OK;
 }
 
@@ -610,9 +636,10 @@ int copy_state(isph_ctx* c, const StateView& a, const StateView& b, int n) {
     HIPCHK(c, hipMemcpyAsync(b.vel, a.vel, n2, D2D, c->stream));
     HIPCHK(c, hipMemcpyAsync(b.sp, a.sp, n2, D2D, c->stream));
     HIPCHK(c, hipMemcpyAsync(b.sv, a.sv, n2, D2D, c->stream));
-    const double* da[] = {a.mass, a.rho, a.oden, a.p, a.por, a.R, a.nut, a.tke, a.varts, a.velm};
-    double* db[] = {b.mass, b.rho, b.oden, b.p, b.por, b.R, b.nut, b.tke, b.varts, b.velm};
-    for (int k = 0; k < 10; ++k) HIPCHK(c, hipMemcpyAsync(db[k], da[k], nd, D2D, c->stream));
+    const double* da[] = {a.mass, a.rho, a.oden, a.p,     a.por, a.R,
+                          a.nut,  a.tke, a.varts, a.velm, a.con, a.ovol};
+    double* db[] = {b.mass, b.rho, b.oden, b.p, b.por, b.R, b.nut, b.tke, b.varts, b.velm, b.con, b.ovol};
+    for (int k = 0; k < 12; ++k) HIPCHK(c, hipMemcpyAsync(db[k], da[k], nd, D2D, c->stream));
     HIPCHK(c, hipMemcpyAsync(b.tid, a.tid, ni, D2D, c->stream));
     HIPCHK(c, hipMemcpyAsync(b.flags, a.flags, ni, D2D, c->stream));
     HIPCHK(c, hipMemcpyAsync(b.pid, a.pid, ni, D2D, c->stream));
@@ -681,14 +708,19 @@ int isph_create(const isph_params* prm, int32_t cap, int32_t cap_soil, isph_ctx*
     rc |= dalloc(c, &c->smass, (size_t)std::max(cap_soil, 1));
     rc |= dalloc(c, &c->srho, (size_t)std::max(cap_soil, 1));
     rc |= dalloc(c, &c->sstart, (size_t)(nb / 2) + 1);
+    rc |= dalloc(c, &c->pos_nb, N);
+    rc |= dalloc(c, &c->cdiff, N);
+    rc |= dalloc(c, &c->salt_coef, N);
     if (hipHostMalloc((void**)&c->sc_host, sizeof(SolverScalars), hipHostMallocDefault) != hipSuccess)
+        rc = ISPH_E_HIP;
+    if (hipHostMalloc((void**)&c->ss_host, sizeof(SimScalars), hipHostMallocDefault) != hipSuccess)
         rc = ISPH_E_HIP;
     if (rc) {
         isph_destroy(c);
         return ISPH_E_HIP;
     }
     memset(c->sc_host, 0, sizeof(SolverScalars));
-    SimScalars ss0 = {0.0, 0.0, prm->dt, 0.0};
+    SimScalars ss0 = {0.0, 0.0, prm->dt, 0.0, prm->dt, 0, 0};
     if (hipMemcpyAsync(c->ss, &ss0, sizeof(ss0), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
         isph_destroy(c);
@@ -706,7 +738,8 @@ int isph_destroy(isph_ctx* c) {
     void* p[] = {c->L, c->divr, c->key, c->key_s, c->rank, c->perm, c->bincnt, c->scanbuf, c->nbr16,
                  c->rq64, c->rv64, c->rq32, c->rv32, c->rsum,
                  c->nbr, c->cnt, c->val64, c->val32, c->dpos, c->rhs, c->part, c->counters,
-                 c->sc, c->ss, c->err, c->spos, c->smass, c->srho, c->sstart};
+                 c->sc, c->ss, c->err, c->spos, c->smass, c->srho, c->sstart,
+                 c->pos_nb, c->cdiff, c->salt_coef};
     for (auto q : p)
         if (q) (void)hipFree(q);
     for (int k = 0; k < 12; ++k)
@@ -714,6 +747,7 @@ int isph_destroy(isph_ctx* c) {
     for (int k = 0; k < 10; ++k)
         if (c->v32[k]) (void)hipFree(c->v32[k]);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
+    if (c->ss_host) (void)hipHostFree(c->ss_host);
     adj_free(c);
     for (auto& sl : c->slots) {
         if (!sl.v.pos) continue;
@@ -761,12 +795,13 @@ int isph_set_particles(isph_ctx* c, int32_t n, const double* x, const double* y,
         return fail(c, ISPH_E_ARG, "null particle array");
     StateView& s = c->st[c->cur];
     std::vector<double2> a(n), b(n), z(n, make_double2(0.0, 0.0));
-    std::vector<double> velm(n), one(n, 1.0), zero(n, 0.0);
+    std::vector<double> velm(n), one(n, 1.0), zero(n, 0.0), ovol(n);
     std::vector<int> fl(n, 0);
     for (int i = 0; i < n; ++i) {
         a[i] = make_double2(x[i], y[i]);
         b[i] = make_double2(vx[i], vy[i]);
         velm[i] = std::sqrt(vx[i] * vx[i] + vy[i] * vy[i]);
+        ovol[i] = mass[i] / oden[i];  // fmass/rho at generation (internal.f90:103)
         if (flags) fl[i] = flags[i];
     }
     const size_t nd = sizeof(double) * n, ni = sizeof(int) * n, n2 = sizeof(double2) * n;
@@ -780,7 +815,8 @@ int isph_set_particles(isph_ctx* c, int32_t n, const double* x, const double* y,
     HIPCHK(c, hipMemcpyAsync(s.oden, oden, nd, H2D, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.p, pressure, nd, H2D, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.por, one.data(), nd, H2D, c->stream));
-    double* zs[] = {s.R, s.nut, s.tke, s.varts};
+    double* zs[] = {s.R, s.nut, s.tke, s.varts, s.con};
+    HIPCHK(c, hipMemcpyAsync(s.ovol, ovol.data(), nd, H2D, c->stream));
     for (auto q : zs) HIPCHK(c, hipMemcpyAsync(q, zero.data(), nd, H2D, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.velm, velm.data(), nd, H2D, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.tid, tid, ni, H2D, c->stream));
@@ -792,7 +828,7 @@ int isph_set_particles(isph_ctx* c, int32_t n, const double* x, const double* y,
     }
     HIPCHK(c, hipMemcpyAsync(c->L, Lh.data(), 4 * nd, H2D, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->divr, zero.data(), nd, H2D, c->stream));
-    SimScalars ss0 = {0.0, 0.0, c->prm.dt, 0.0};
+    SimScalars ss0 = {0.0, 0.0, c->prm.dt, 0.0, c->prm.dt, 0, 0};
     HIPCHK(c, hipMemcpyAsync(c->ss, &ss0, sizeof(ss0), H2D, c->stream));
     c->n = n;
     int rc = sort_particles(c, false);
@@ -904,6 +940,11 @@ static double* field_ptr(isph_ctx* c, int32_t f, int* width, bool* pair, int* co
         case ISPH_FIELD_VY: *pair = true; *comp = 1; return (double*)s.vel;
         case ISPH_FIELD_DENSITY: return s.rho;
         case ISPH_FIELD_PRESSURE: return s.p;
+        case ISPH_FIELD_CON: return s.con;
+        case ISPH_FIELD_OVOL: return s.ovol;
+        case ISPH_FIELD_CDIFF: return c->cdiff;
+        case ISPH_FIELD_MASS: return s.mass;
+        case ISPH_FIELD_ODEN: return s.oden;
         default: return nullptr;
     }
 }
@@ -1055,6 +1096,7 @@ STAGE(isph_resetid, resetid_)
 STAGE(isph_neighbour, run_neighbour)
 STAGE(isph_effpor, run_effpor)
 STAGE(isph_eddyvis, run_eddyvis)
+STAGE(isph_eddyvis_lagged, run_eddyvis_lagged)
 STAGE(isph_int_vel, run_int_vel)
 STAGE(isph_ppe_assemble, run_assemble)
 STAGE(isph_comp_vel, run_comp_vel)

@@ -19,6 +19,7 @@ constexpr double C_HYB2 = 9.4200000762939453125;  // float32(3.0*3.14),  kernel.
 constexpr double ALP = 0.1;                       // kernel.f90:30
 constexpr double TINY_DET = 9.999999747378751635551453e-06;  // float32(1e-5), functions.f90:75
 constexpr double TINY6 = 9.999999974752427078783512e-07;     // float32(1e-6), integrator.f90:497
+constexpr double DM_SALT = 9.9999997171806853657471947e-10;  // float32(1e-9), scalar.f90:118
 
 // Kernel constants derived once per launch from the parameters.
 struct KernelConsts {

@@ -65,7 +65,11 @@ struct isph_ctx {
     isph::SolverScalars* sc = nullptr;
     isph::SolverScalars* sc_host = nullptr;  // pinned
     isph::SimScalars* ss = nullptr;
+    isph::SimScalars* ss_host = nullptr;  // pinned
     uint32_t* err = nullptr;
+    double2* pos_nb = nullptr;     // positions at the last neighbour search
+    double* cdiff = nullptr;       // salinity rate of the last scalart
+    double2* salt_coef = nullptr;  // per-particle dispersion (Dx, Dy)
     double2* spos = nullptr;
     double *smass = nullptr, *srho = nullptr;
     int* sstart = nullptr;
@@ -122,6 +126,7 @@ int run_projection(isph_ctx* c);
 int run_neighbour(isph_ctx* c);
 int run_effpor(isph_ctx* c);
 int run_eddyvis(isph_ctx* c);
+int run_eddyvis_lagged(isph_ctx* c);  // pair geometry of the last search
 int run_int_vel(isph_ctx* c);
 int run_assemble(isph_ctx* c);
 int run_solve(isph_ctx* c);
@@ -139,6 +144,12 @@ int solve_fp64_on(isph_ctx* c, const double* val, const double* rhs, double* x);
 
 // discrete adjoint (isph_adjoint.hip)
 void adj_free(isph_ctx* c);
+
+// salinity transport (isph_scalar.hip)
+int run_scalart(isph_ctx* c);
+int run_scalarupdate(isph_ctx* c, double stime, bool from_ss);
+int run_densityupdate(isph_ctx* c);
+int run_salt_substeps(isph_ctx* c);
 
 }  // namespace host
 }  // namespace isph

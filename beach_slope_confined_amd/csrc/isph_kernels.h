@@ -17,6 +17,11 @@ constexpr uint32_t ERR_NAN = 4u;
 constexpr uint32_t ERR_SINGULAR = 8u;
 constexpr uint32_t ERR_NOCONV = 16u;
 constexpr uint32_t ERR_INDEX16 = 32u;  // a column offset does not fit 16 bits
+constexpr uint32_t ERR_SUBSTEPS = 64u; // salinity transport needs > MAX_SUBSTEPS substeps
+
+// cap on the transport substeps of one step (integrator.f90:966-970 has none;
+// its ceiling() would overflow a default integer long before this)
+constexpr int MAX_SUBSTEPS = 1 << 20;
 
 // solver `done` states
 constexpr int DONE_RUNNING = 0;
@@ -42,11 +47,15 @@ struct PorousParams {
 struct StateView {
     double2 *pos, *vel, *sp, *sv;  // (x,y), (vx,vy), (xs,ys), (vxs,vys)
     double *mass, *rho, *oden, *p, *por, *R, *nut, *tke, *varts, *velm;
+    double *con, *ovol;            // salinity and particle volume (scalar.f90)
     int *tid, *flags, *pid;
 };
 
 struct SimScalars {
     double umax, numax, dt, t;
+    double dtsol;   // transport substep (integrator.f90:964-972)
+    int solsteps;   // transport substeps per step
+    int pad;
 };
 
 struct SolverScalars {
@@ -255,8 +264,25 @@ __global__ void k_shift(int n, KernelConsts kc, double csh, double cap, size_t l
 __global__ void k_shift_apply(int n, StateView s);
 __global__ void k_maxes(int n, const double* velm, const double* nut, double* part);
 __global__ void k_timestep(int nb, const double* part, SimScalars* ss, double dl, double s1,
-                           double mu, double rho, int adaptive);
+                           double mu, double rho, int adaptive, double tschmidt, int salinity,
+                           uint32_t* err);
 __global__ void k_advance_time(SimScalars* ss);
 __global__ void k_copy(const double2* a, double2* b, size_t n);
+
+// salinity transport and variable density (isph_scalar.hip)
+struct SaltConsts {
+    double lam, dl, tschmidt;
+    double rhomax, rhomin, xbuf;
+};
+__global__ void k_scalar_coef(int n, SaltConsts sc, StateView s, const SimScalars* ss,
+                              double2* coef);
+__global__ void k_scalart(int n, KernelConsts kh, SaltConsts sc, size_t ld, StateView s,
+                          const double2* pos_nb, const int* nbr, const int* cnt,
+                          const double2* coef, double* cdiff);
+__global__ void k_scalarupdate(int n, SaltConsts sc, StateView s, const double* cdiff,
+                               double stime, const SimScalars* ss);
+__global__ void k_densityupdate(int n, SaltConsts sc, StateView s);
+__global__ void k_salt_init(int n, StateView s, double xl, double yl, double line_grad,
+                            double rel_den);
 
 }  // namespace isph

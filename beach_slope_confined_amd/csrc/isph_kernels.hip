@@ -126,6 +126,8 @@ __global__ void k_gather_state(int n, const int* __restrict__ perm, StateView sr
     dst.tke[i] = src.tke[o];
     dst.varts[i] = src.varts[o];
     dst.velm[i] = src.velm[o];
+    dst.con[i] = src.con[o];
+    dst.ovol[i] = src.ovol[o];
     dst.tid[i] = src.tid[o];
     dst.flags[i] = src.flags[o];
     dst.pid[i] = src.pid[o];
@@ -691,7 +693,8 @@ __global__ void k_maxes(int n, const double* __restrict__ velm, const double* __
 }
 
 __global__ void k_timestep(int nb, const double* __restrict__ part, SimScalars* ss, double dl,
-                           double s1, double mu, double rho, int adaptive) {
+                           double s1, double mu, double rho, int adaptive, double tschmidt,
+                           int salinity, uint32_t* err) {
     double a = 0.0, b = 0.0;
     for (int k = threadIdx.x; k < nb; k += BLOCK) {
         a = fmax(a, part[k]);
@@ -724,6 +727,25 @@ __global__ void k_timestep(int nb, const double* __restrict__ part, SimScalars* 
         }
         ss->dt = dt;
     }
+    // transport substeps (integrator.f90:961-972); Dm is the float32 literal 1e-9
+    double nm = numax;
+    if (!isfinite(nm)) nm = 0.0;
+    ss->numax = nm;
+    const double dt = ss->dt;
+    double dts = (s1 * (dl * dl)) / (DM_SALT + nm / tschmidt);
+    int steps = 1;
+    if (dts < dt) {
+        const double q = ceil(dt / dts);
+        if (!(q <= (double)MAX_SUBSTEPS)) {
+            steps = 0;
+            if (salinity) raise_err(err, ERR_SUBSTEPS);
+        } else {
+            steps = (int)q;
+            dts = dt / (double)steps;
+        }
+    }
+    ss->dtsol = (dts < dt) ? dts : dt;
+    ss->solsteps = steps;
 }
 
 __global__ void k_advance_time(SimScalars* ss) {

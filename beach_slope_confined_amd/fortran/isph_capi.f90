@@ -20,9 +20,9 @@ module isph_capi
     type, bind(C) :: isph_params
         real(c_double) :: h, cell_size, xmin, ymin, xmax, ymax, dl, lam, dt, rho0, rel_den, g, nu
         real(c_double) :: csh, maxshift, lamfs, maxdivr, por, alpha, beta, d50, tol, inner_tol
-        real(c_double) :: xlcutoff, ytcutoff, sig1, mu
+        real(c_double) :: xlcutoff, ytcutoff, sig1, mu, rhomin, tschmidt, salt_xbuf
         integer(c_int32_t) :: kernel, drag_law, precision, maxit, inner_maxit, max_outer, max_nbr
-        integer(c_int32_t) :: shifting, adaptive_dt, sink, visc_law, reserved
+        integer(c_int32_t) :: shifting, adaptive_dt, sink, visc_law, salinity
     end type isph_params
 
     interface
@@ -142,6 +142,37 @@ module isph_capi
             import :: c_int, c_int32_t, c_ptr
             type(c_ptr), value :: ctx
             integer(c_int32_t), value :: nsteps
+        end function
+        ! salinity transport and variable density (scalar.f90, main.f90:160-263)
+        integer(c_int) function isph_scalart(ctx) bind(C, name="isph_scalart")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function isph_scalarupdate(ctx, stime) bind(C, name="isph_scalarupdate")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: stime
+        end function
+        integer(c_int) function isph_densityupdate(ctx) bind(C, name="isph_densityupdate")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function isph_eddyvis_lagged(ctx) bind(C, name="isph_eddyvis_lagged")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function isph_salt_init(ctx, xl, yl, line_grad) &
+                                               bind(C, name="isph_salt_init")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: xl, yl, line_grad
+        end function
+        integer(c_int) function isph_get_substeps(ctx, solsteps, dtsol) &
+                                                  bind(C, name="isph_get_substeps")
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), intent(out) :: solsteps
+            real(c_double), intent(out) :: dtsol
         end function
     end interface
 

@@ -27,6 +27,7 @@ FIELD = {
     "xs": 0, "ys": 1, "vxs": 2, "vys": 3, "porosity": 4, "resist": 5,
     "coff": 6, "divr": 7, "nut": 8, "tke": 9, "varts": 10, "vel": 11, "rhs": 12,
     "x": 13, "y": 14, "vx": 15, "vy": 16, "density": 17, "pressure": 18,
+    "con": 19, "ovol": 20, "cdiff": 21, "mass": 22, "oden": 23,
 }
 
 OK = 0
@@ -39,10 +40,11 @@ _DOUBLES = [
     "h", "cell_size", "xmin", "ymin", "xmax", "ymax", "dl", "lam", "dt", "rho0",
     "rel_den", "g", "nu", "csh", "maxshift", "lamfs", "maxdivr", "por", "alpha",
     "beta", "d50", "tol", "inner_tol", "xlcutoff", "ytcutoff", "sig1", "mu",
+    "rhomin", "tschmidt", "salt_xbuf",
 ]
 _INTS = [
     "kernel", "drag_law", "precision", "maxit", "inner_maxit", "max_outer",
-    "max_nbr", "shifting", "adaptive_dt", "sink", "visc_law", "reserved",
+    "max_nbr", "shifting", "adaptive_dt", "sink", "visc_law", "salinity",
 ]
 
 
@@ -68,7 +70,8 @@ assert ctypes.sizeof(IsphParams) == 8 * len(_DOUBLES) + 4 * len(_INTS)
 def make_params(**kw):
     """Parameters with the reference's constants where the configs do not set
     them (``initialize.f90:21-43``: csh 0.5, maxshift 0.1, lamfs 0.8,
-    maxdivr 2, rel_den 1.025, sig1 0.2, mu 1e-3)."""
+    maxdivr 2, rel_den 1.025, sig1 0.2, mu 1e-3, rhomin 1000, tschmidt 1;
+    the buffer reset line x < 1.6 of ``scalar.f90:252``)."""
     p = IsphParams()
     defaults = dict(
         kernel=KERNEL_WENDLAND_C2, drag_law=DRAG_ERGUN, visc_law=VISC_CONSTANT,
@@ -76,7 +79,8 @@ def make_params(**kw):
         csh=0.5, maxshift=0.1, lamfs=0.8, maxdivr=2.0, por=0.4, alpha=200.0,
         beta=1.1, d50=0.01, tol=1e-8, inner_tol=0.1, maxit=50000,
         inner_maxit=0, max_outer=200, max_nbr=64, shifting=1, adaptive_dt=0,
-        sink=0, xlcutoff=-1e300, ytcutoff=1e300, sig1=0.2, mu=1e-3, reserved=0,
+        sink=0, xlcutoff=-1e300, ytcutoff=1e300, sig1=0.2, mu=1e-3, salinity=0,
+        rhomin=1000.0, tschmidt=1.0, salt_xbuf=1.6,
     )
     defaults.update(kw)
     for k, v in defaults.items():

@@ -100,6 +100,11 @@ typedef struct isph_params {
     double ytcutoff;
     double sig1;       /* adaptive dt factor (initialize.f90:35)               */
     double mu;         /* dynamic viscosity used by adaptive dt (initialize.f90:35) */
+    double rhomin;     /* fresh-water density of the salinity law (initialize.f90:22);
+                          rhomax = rho0 * rel_den (setup.f90:19)               */
+    double tschmidt;   /* turbulent Schmidt number (initialize.f90:23)         */
+    double salt_xbuf;  /* buffer particles with x < salt_xbuf are reset to
+                          con = 0.5 after each transport substep (scalar.f90:252) */
     int32_t kernel;      /* ISPH_KERNEL_*                                       */
     int32_t drag_law;    /* ISPH_DRAG_*                                         */
     int32_t precision;   /* ISPH_PREC_*                                         */
@@ -111,7 +116,9 @@ typedef struct isph_params {
     int32_t adaptive_dt; /* 1: isph_timestep() updates dt (integrator.f90:950-960) */
     int32_t sink;        /* 1: isph_cellshiftalt() deletes outflow particles    */
     int32_t visc_law;    /* 0: mu = vis(oden) (porous.f90:12-30); 1: mu = nu*oden */
-    int32_t reserved;    /* keeps the struct a multiple of 8 bytes; set to 0    */
+    int32_t salinity;    /* 1: isph_step() runs the salt-water step of main.f90:199-263
+                            (transport substeps first, densityupdate after the
+                            shift, eddyvis after timestep); 0: main.f90:41-66    */
 } isph_params;
 
 typedef struct isph_ctx isph_ctx;
@@ -136,6 +143,11 @@ typedef struct isph_ctx isph_ctx;
 #define ISPH_FIELD_VY      16   /* velocity y                                   */
 #define ISPH_FIELD_DENSITY 17   /* density rho = n * oden (porous.f90:190)       */
 #define ISPH_FIELD_PRESSURE 18  /* pressure (isph.f90:465-483)                  */
+#define ISPH_FIELD_CON     19   /* salinity con in [-0.5, 0.5] (particle.f90:37)  */
+#define ISPH_FIELD_OVOL    20   /* particle volume fmass/rho (internal.f90:103)   */
+#define ISPH_FIELD_CDIFF   21   /* d con/dt of the last scalart (scalar.f90:129) */
+#define ISPH_FIELD_MASS    22   /* mass                                          */
+#define ISPH_FIELD_ODEN    23   /* fluid density oden (scalar.f90:293)           */
 
 /* ---- context ------------------------------------------------------------ */
 /* Allocates every device buffer for up to cap_particles particles and
@@ -192,6 +204,11 @@ int isph_resetid(isph_ctx* ctx);       /* isph.f90:88-113                    */
 int isph_neighbour(isph_ctx* ctx);     /* search.f90:310-399 + nsearch2 :13-308 */
 int isph_effpor(isph_ctx* ctx);        /* porous.f90:107-224 (+ porsearch :32) */
 int isph_eddyvis(isph_ctx* ctx);       /* turbulence.f90:12-110              */
+/* eddyvis with the pair distances and kernel gradients of the last
+ * isph_neighbour() (the reference's stored klt, turbulence.f90:36) instead
+ * of the current positions: the eddyvis of the salt-water step, which
+ * main.f90:260 calls after opt2_shift has moved the particles. */
+int isph_eddyvis_lagged(isph_ctx* ctx);
 int isph_int_vel(isph_ctx* ctx);       /* integrator.f90:458-713             */
 /* ppesolve = assembly (isph.f90:123-456) + solve (:460) + scatter (:465-483) */
 int isph_ppe_assemble(isph_ctx* ctx);
@@ -204,8 +221,28 @@ int isph_freesurf(isph_ctx* ctx);      /* isph.f90:49-86                     */
 int isph_opt2_shift(isph_ctx* ctx);    /* part_shift.f90:12-192              */
 int isph_timestep(isph_ctx* ctx);      /* integrator.f90:908-976             */
 /* nsteps whole steps in the main.f90:41-66 order (no host round trip
- * between stages). */
+ * between stages); with params.salinity = 1 in the order of main.f90:199-263
+ * (without the buffer insertion of :217-235). */
 int isph_step(isph_ctx* ctx, int32_t nsteps);
+
+/* ---- salinity transport and variable density (scalar.f90) --------------
+ * con is the salinity of a fluid particle, -0.5 (fresh) to 0.5 (salt); the
+ * fluid density follows it linearly between rhomin and rhomax.  The
+ * transport uses the neighbour lists, pair distances and porosity of the
+ * last isph_neighbour()/isph_effpor() with the current positions and
+ * velocities, as the reference's stored lists do when scalart runs at the
+ * start of a step.  isph_set_particles() sets con = 0 and ovol = mass/oden. */
+int isph_scalart(isph_ctx* ctx);                  /* scalar.f90:114-236: cdiff */
+int isph_scalarupdate(isph_ctx* ctx, double stime); /* scalar.f90:238-270        */
+int isph_densityupdate(isph_ctx* ctx);            /* scalar.f90:272-305        */
+/* Salt-water start of main.f90:160-195: every fluid particle gets
+ * con = -0.5; those above the line (y - yl) - line_grad (x - xl) > 0 get
+ * mass, oden and density times rel_den and con = 0.5. */
+int isph_salt_init(isph_ctx* ctx, double xl, double yl, double line_grad);
+/* Transport substeps of the next step and their length, as the last
+ * isph_timestep() set them (integrator.f90:961-972); 0 substeps before the
+ * first timestep, as the reference's solsteps = 0 (initialize.f90:56). */
+int isph_get_substeps(isph_ctx* ctx, int32_t* solsteps, double* dtsol);
 
 /* ---- in-memory checkpoints (no reference counterpart: the reference has no
  * restart, output.f90 only writes text snapshots) ---------------------------- */

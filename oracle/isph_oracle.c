@@ -45,6 +45,8 @@
 #define ALP 0.1                           /* kernel.f90:30 */
 #define TINY_DET 9.999999747378751635551453e-06  /* 1e-5, functions.f90:75 */
 #define TINY6 9.999999974752427078783512e-07     /* 1e-6, integrator.f90:497,582 */
+#define DM_SALT 9.9999997171806853657471947e-10   /* 1e-9, scalar.f90:118, integrator.f90:914 */
+#define MAX_SUBSTEPS (1 << 20)             /* cap on transport substeps per step */
 #define RBLOCK 256                        /* reduction block of the GPU kernels */
 
 struct ora_ctx {
@@ -52,6 +54,8 @@ struct ora_ctx {
     int32_t cap, n, cap_soil, ns;
     double *x, *y, *vx, *vy, *xs, *ys, *vxs, *vys, *mass, *rho, *oden, *p;
     double *por, *R, *nut, *tke, *varts, *vel, *L, *divr;
+    double *con, *ovol, *cdiff;   /* salinity, particle volume, d con/dt */
+    double *xnb, *ynb;            /* positions at the last neighbour search */
     int32_t *tid, *flags, *pid;
     /* cell grid */
     int32_t nx, ny;
@@ -73,8 +77,8 @@ struct ora_ctx {
     int assembled;
     double *w[10];
     float *f[8];
-    double umax, numax, dt, t, relres;
-    int32_t iters, outer;
+    double umax, numax, dt, t, relres, dtsol;
+    int32_t iters, outer, solsteps;
     char err[256];
 };
 
@@ -301,7 +305,8 @@ int ora_create(const isph_params* prm, int32_t cap, int32_t cap_soil, ora_ctx** 
     size_t nd = sizeof(double) * (size_t)cap;
     double** dp[] = {&c->x, &c->y, &c->vx, &c->vy, &c->xs, &c->ys, &c->vxs, &c->vys,
                      &c->mass, &c->rho, &c->oden, &c->p, &c->por, &c->R, &c->nut,
-                     &c->tke, &c->varts, &c->vel, &c->divr, &c->tmpd, &c->rhs};
+                     &c->tke, &c->varts, &c->vel, &c->divr, &c->tmpd, &c->rhs,
+                     &c->con, &c->ovol, &c->cdiff, &c->xnb, &c->ynb};
     for (size_t k = 0; k < sizeof(dp) / sizeof(dp[0]); ++k) *dp[k] = (double*)xmalloc(nd);
     c->L = (double*)xmalloc(4 * nd);
     for (int k = 0; k < 10; ++k) c->w[k] = (double*)xmalloc(nd);
@@ -328,6 +333,7 @@ int ora_create(const isph_params* prm, int32_t cap, int32_t cap_soil, ora_ctx** 
     c->sm = (double*)xmalloc(ns);
     c->sr = (double*)xmalloc(ns);
     c->dt = prm->dt;
+    c->dtsol = prm->dt;
     *out = c;
     return ISPH_OK;
 }
@@ -337,7 +343,7 @@ int ora_destroy(ora_ctx* c) {
     double* dp[] = {c->x, c->y, c->vx, c->vy, c->xs, c->ys, c->vxs, c->vys, c->mass,
                     c->rho, c->oden, c->p, c->por, c->R, c->nut, c->tke, c->varts,
                     c->vel, c->divr, c->tmpd, c->rhs, c->L, c->val, c->sx, c->sy,
-                    c->sm, c->sr};
+                    c->sm, c->sr, c->con, c->ovol, c->cdiff, c->xnb, c->ynb};
     for (size_t k = 0; k < sizeof(dp) / sizeof(dp[0]); ++k) free(dp[k]);
     for (int k = 0; k < 10; ++k) free(c->w[k]);
     for (int k = 0; k < 8; ++k) free(c->f[k]);
@@ -379,7 +385,7 @@ static inline int32_t cell_of(const ora_ctx* c, double x, double y, int* oob) {
 static void apply_perm(ora_ctx* c, int32_t nnew) {
     double* dp[] = {c->x, c->y, c->vx, c->vy, c->xs, c->ys, c->vxs, c->vys, c->mass,
                     c->rho, c->oden, c->p, c->por, c->R, c->nut, c->tke, c->varts,
-                    c->vel};
+                    c->vel, c->con, c->ovol};
     for (size_t k = 0; k < sizeof(dp) / sizeof(dp[0]); ++k) {
         double* a = dp[k];
         for (int32_t i = 0; i < nnew; ++i) c->tmpd[i] = a[c->perm[i]];
@@ -453,11 +459,14 @@ int ora_set_particles(ora_ctx* c, int32_t n, const double* x, const double* y,
         c->xs[i] = 0.0; c->ys[i] = 0.0; c->vxs[i] = 0.0; c->vys[i] = 0.0;
         c->por[i] = 1.0; c->R[i] = 0.0; c->nut[i] = 0.0; c->tke[i] = 0.0;
         c->varts[i] = 0.0;
+        c->con[i] = 0.0;
+        c->ovol[i] = mass[i] / oden[i];  /* fmass/rho, internal.f90:103 */
         c->vel[i] = sqrt(vx[i] * vx[i] + vy[i] * vy[i]);
         c->divr[i] = 0.0;
         c->L[4 * i] = 1.0; c->L[4 * i + 1] = 0.0; c->L[4 * i + 2] = 0.0; c->L[4 * i + 3] = 1.0;
     }
     c->umax = 0.0; c->numax = 0.0; c->t = 0.0; c->dt = c->prm.dt;
+    c->dtsol = c->prm.dt; c->solsteps = 0;  /* solsteps = 0, initialize.f90:56 */
     return sort_particles(c, 0);
 }
 
@@ -523,6 +532,17 @@ static double* field_ptr(ora_ctx* c, int32_t f, int* width) {
         case ISPH_FIELD_VARTS: return c->varts;
         case ISPH_FIELD_VEL: return c->vel;
         case ISPH_FIELD_RHS: return c->rhs;
+        case ISPH_FIELD_X: return c->x;
+        case ISPH_FIELD_Y: return c->y;
+        case ISPH_FIELD_VX: return c->vx;
+        case ISPH_FIELD_VY: return c->vy;
+        case ISPH_FIELD_DENSITY: return c->rho;
+        case ISPH_FIELD_PRESSURE: return c->p;
+        case ISPH_FIELD_CON: return c->con;
+        case ISPH_FIELD_OVOL: return c->ovol;
+        case ISPH_FIELD_CDIFF: return c->cdiff;
+        case ISPH_FIELD_MASS: return c->mass;
+        case ISPH_FIELD_ODEN: return c->oden;
         default: return NULL;
     }
 }
@@ -657,6 +677,11 @@ int ora_neighbour(ora_ctx* c) {
         }
         c->flags[i] = fl;
     }
+    /* the reference stores dist and klt per pair (search.f90:65-74); keep
+     * the positions they were computed at for stages that run after the
+     * shift has moved the particles */
+    memcpy(c->xnb, c->x, sizeof(double) * (size_t)c->n);
+    memcpy(c->ynb, c->y, sizeof(double) * (size_t)c->n);
     c->have_nbr = 1;
     c->assembled = 0;
     if (over) return set_err(c, ISPH_E_CAPACITY, "neighbour list capacity exceeded");
@@ -724,10 +749,18 @@ static inline void pair_geom(const ora_ctx* c, int32_t i, int32_t j, double* r,
     ora_kernel_gradw(&c->prm, *r, dx, dy, wx, wy);
 }
 
+/* r and grad W of pair (i, j) as stored by the last neighbour search */
+static inline void pair_geom_nb(const ora_ctx* c, int32_t i, int32_t j, double* r,
+                                double* wx, double* wy) {
+    double dx = c->xnb[i] - c->xnb[j], dy = c->ynb[i] - c->ynb[j];
+    *r = sqrt(dx * dx + dy * dy);
+    ora_kernel_gradw(&c->prm, *r, dx, dy, wx, wy);
+}
+
 /* ---------------------------------------------------------------------- */
 /* turbulence.f90:12-110 eddyvis                                           */
 /* ---------------------------------------------------------------------- */
-int ora_eddyvis(ora_ctx* c) {
+static int eddyvis_core(ora_ctx* c, int stored) {
     const isph_params* prm = &c->prm;
     const int32_t M = prm->max_nbr;
     const double cs = 0.150, cv = 0.080, ce = 1.0;
@@ -742,7 +775,8 @@ int ora_eddyvis(ora_ctx* c) {
         for (int32_t k = 0; k < c->nptr[i]; ++k) {
             int32_t j = c->nidx[(size_t)i * M + k];
             double r, wx, wy;
-            pair_geom(c, i, j, &r, &wx, &wy);
+            if (stored) pair_geom_nb(c, i, j, &r, &wx, &wy);
+            else pair_geom(c, i, j, &r, &wx, &wy);
             double gx = L[0] * wx + L[1] * wy;
             double gy = L[2] * wx + L[3] * wy;
             double du = c->vx[j] / c->por[j] - ui;
@@ -761,6 +795,11 @@ int ora_eddyvis(ora_ctx* c) {
 /* ---------------------------------------------------------------------- */
 /* integrator.f90:458-713 int_vel                                          */
 /* ---------------------------------------------------------------------- */
+int ora_eddyvis(ora_ctx* c) { return eddyvis_core(c, 0); }
+/* eddyvis on the stored pair geometry of the last search (the reference's
+ * klt), as main.f90:260 calls it after the shift */
+int ora_eddyvis_lagged(ora_ctx* c) { return eddyvis_core(c, 1); }
+
 int ora_int_vel(ora_ctx* c) {
     const isph_params* prm = &c->prm;
     const int32_t M = prm->max_nbr;
@@ -1455,14 +1494,168 @@ int ora_timestep(ora_ctx* c) {
         }
         c->dt = dt;
     }
+    /* transport substeps, integrator.f90:961-972 */
+    if (!isfinite(numax)) c->numax = 0.0;
+    {
+        const double dl = prm->dl, dt = c->dt;
+        double dts = (prm->sig1 * (dl * dl)) / (DM_SALT + c->numax / prm->tschmidt);
+        int32_t steps = 1;
+        if (dts < dt) {
+            double q = ceil(dt / dts);
+            if (!(q <= (double)MAX_SUBSTEPS)) {
+                steps = 0;
+                if (prm->salinity) {
+                    c->dtsol = (dts < dt) ? dts : dt;
+                    c->solsteps = 0;
+                    return set_err(c, ISPH_E_CAPACITY, "salinity transport needs more than 2^20 substeps");
+                }
+            } else {
+                steps = (int32_t)q;
+                dts = dt / (double)steps;
+            }
+        }
+        c->dtsol = (dts < dt) ? dts : dt;
+        c->solsteps = steps;
+    }
     return ISPH_OK;
 }
 
-/* One step in the order of main.f90:41-66 (t advances first, :42-43). */
+/* ---------------------------------------------------------------------- */
+/* scalar.f90 salinity transport and variable density                      */
+/* ---------------------------------------------------------------------- */
+static int salt_live(const ora_ctx* c, int32_t i) {
+    return c->tid[i] == 3 && !(c->flags[i] & ISPH_FLAG_DEAD);
+}
+
+/* scalar.f90:114-236 scalart: cdiff of every live fluid particle, pairwise
+ * over its stored list.  As in the reference, the dispersion of both
+ * particles is evaluated per pair (:138-154, :160-170), the pair distance is
+ * the stored dist(m) (positions of the last neighbour search) and the
+ * position differences are current (:214-229). */
+int ora_scalart(ora_ctx* c) {
+    const isph_params* prm = &c->prm;
+    const int32_t M = prm->max_nbr;
+    const double al = 0.0025, at = 0.00025, Dm = DM_SALT, tor = 1.0;
+    const double dl = prm->dl, dt = c->dt, lam = prm->lam, ts = prm->tschmidt;
+    const double cap = (0.12 * (dl * dl)) / dt;
+    isph_params hyb = *prm;
+    hyb.kernel = ISPH_KERNEL_HYBRID;   /* hWabx/hWaby, kernel.f90:204-248 */
+    if (!c->have_nbr) return set_err(c, ISPH_E_STATE, "scalart before neighbour");
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < c->n; ++i) {
+        if (!salt_live(c, i)) continue;
+        double cdiff = 0.0;
+        const double con1 = c->con[i];
+        const double vxi = c->vx[i], vyi = c->vy[i], pori = c->por[i];
+        const double vi = sqrt(vxi * vxi + vyi * vyi);
+        double Dxi = Dm + ((al * (vxi * vxi) + at * (vyi * vyi)) + ((al - at) * vxi) * vyi) / (vi * pori);
+        Dxi = (Dxi > 0.0) ? Dxi : Dm;
+        Dxi = (Dxi <= cap) ? Dxi : cap;
+        double Dyi = Dm + ((at * (vxi * vxi) + al * (vyi * vyi)) + ((al - at) * vxi) * vyi) / (vi * pori);
+        Dyi = (Dyi > 0.0) ? Dyi : Dm;
+        Dyi = (Dyi <= cap) ? Dyi : cap;
+        for (int32_t k = 0; k < c->nptr[i]; ++k) {
+            int32_t j = c->nidx[(size_t)i * M + k];
+            if (!salt_live(c, j)) continue;
+            const double con2 = c->con[j];
+            const double vxj = c->vx[j], vyj = c->vy[j], porj = c->por[j];
+            const double vj = sqrt(vxj * vxj + vyj * vyj);
+            double Dxj = Dm + ((al * (vxj * vxj) + at * (vyj * vyj)) + ((al - at) * vxj) * vyj) / (vj * porj);
+            double Dyj = Dm + ((at * (vxj * vxj) + al * (vyj * vyj)) + ((al - at) * vxj) * vyj) / (vj * porj);
+            Dxj = (Dxj > 0.0) ? Dxj : Dm;
+            Dxj = (Dxj <= cap) ? Dxj : cap;
+            Dyj = (Dyj > 0.0) ? Dyj : Dm;
+            Dyj = (Dyj <= cap) ? Dyj : cap;
+            double t1x, t1y;
+            if (pori >= 0.8 || porj >= 0.8) {
+                if (c->nut[i] < TINY6 && c->nut[j] < TINY6) {
+                    t1x = Dm + ((c->nut[i] + c->nut[j]) * 0.50) / ts;
+                } else {
+                    t1x = 2.0 * (((Dm + c->nut[i] / ts) * (Dm + c->nut[j] / ts)) /
+                                 ((Dm + c->nut[i] / ts) + (Dm + c->nut[j] / ts)));
+                }
+                t1y = t1x;
+            } else {
+                t1x = (((2.0 * tor) * Dxi) * Dxj) / (Dxi + Dxj);
+                t1y = (((2.0 * tor) * Dyi) * Dyj) / (Dyi + Dyj);
+            }
+            const double t2 = 1.0;
+            const double t3 = (c->ovol[i] * (pori + porj)) / (pori * porj);
+            const double ex = c->xnb[i] - c->xnb[j], ey = c->ynb[i] - c->ynb[j];
+            const double r = sqrt(ex * ex + ey * ey);
+            const double dx = c->x[i] - c->x[j], dy = c->y[i] - c->y[j];
+            double hx, hy;
+            ora_kernel_gradw(&hyb, r, dx, dy, &hx, &hy);
+            cdiff = (cdiff + (((((t1x * t2) * t3) * (con1 - con2)) * dx) * hx) / (r * r + lam)) +
+                    (((((t1y * t2) * t3) * (con1 - con2)) * dy) * hy) / (r * r + lam);
+        }
+        c->cdiff[i] = cdiff;
+    }
+    return ISPH_OK;
+}
+
+/* scalar.f90:238-270 scalarupdate */
+int ora_scalarupdate(ora_ctx* c, double stime) {
+    const double xbuf = c->prm.salt_xbuf;
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < c->n; ++i) {
+        if (!salt_live(c, i)) continue;
+        c->con[i] = c->con[i] + c->cdiff[i] * stime;
+        if ((c->flags[i] & ISPH_FLAG_BUFFER) && c->x[i] < xbuf) c->con[i] = 0.5;
+    }
+    return ISPH_OK;
+}
+
+/* scalar.f90:272-305 densityupdate (rhomax = rho rel_den, setup.f90:19) */
+int ora_densityupdate(ora_ctx* c) {
+    const double rhomax = c->prm.rho0 * c->prm.rel_den, rhomin = c->prm.rhomin;
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < c->n; ++i) {
+        if (!salt_live(c, i)) continue;
+        c->oden[i] = 0.5 * (rhomax + rhomin) + c->con[i] * (rhomax - rhomin);
+        c->mass[i] = c->oden[i] * c->ovol[i];
+        c->rho[i] = c->oden[i] * c->por[i];
+    }
+    return ISPH_OK;
+}
+
+/* main.f90:160-195: salt-water start */
+int ora_salt_init(ora_ctx* c, double xl, double yl, double line_grad) {
+    const double rel = c->prm.rel_den;
+    for (int32_t i = 0; i < c->n; ++i) {
+        if (c->tid[i] != 3) continue;
+        c->con[i] = -0.50;
+        if (((c->y[i] - yl) - line_grad * (c->x[i] - xl)) > 0.0) {
+            c->mass[i] = c->mass[i] * rel;
+            c->oden[i] = c->oden[i] * rel;
+            c->rho[i] = c->rho[i] * rel;
+            c->con[i] = 0.50;
+        }
+    }
+    return ISPH_OK;
+}
+
+int ora_get_substeps(ora_ctx* c, int32_t* solsteps, double* dtsol) {
+    if (solsteps) *solsteps = c->solsteps;
+    if (dtsol) *dtsol = c->dtsol;
+    return ISPH_OK;
+}
+
+/* One step in the order of main.f90:41-66 (t advances first, :42-43); with
+ * salinity in the order of main.f90:199-263 without the buffers of :217-235:
+ * the transport substeps on the previous step's lists first (:204-215),
+ * densityupdate after the shift (:258), eddyvis after timestep (:260). */
 int ora_step(ora_ctx* c, int32_t nsteps) {
     int rc;
+    const int salt = c->prm.salinity != 0;
     for (int32_t s = 0; s < nsteps; ++s) {
         c->t = c->t + c->dt;
+        if (salt && c->have_nbr) {
+            for (int32_t k = 0; k < c->solsteps; ++k) {
+                if ((rc = ora_scalart(c))) return rc;
+                if ((rc = ora_scalarupdate(c, c->dtsol))) return rc;
+            }
+        }
         if ((rc = ora_projection(c))) return rc;
         if ((rc = ora_cellshift(c))) return rc;
         if ((rc = ora_resetid(c))) return rc;
@@ -1478,7 +1671,9 @@ int ora_step(ora_ctx* c, int32_t nsteps) {
         if ((rc = ora_freesurf(c))) return rc;
         if ((rc = ora_effpor(c))) return rc;
         if ((rc = ora_opt2_shift(c))) return rc;
+        if (salt && (rc = ora_densityupdate(c))) return rc;
         if ((rc = ora_timestep(c))) return rc;
+        if (salt && (rc = ora_eddyvis_lagged(c))) return rc;
     }
     return ISPH_OK;
 }

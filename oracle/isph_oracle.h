@@ -66,6 +66,12 @@ int ora_freesurf(ora_ctx* c);
 int ora_opt2_shift(ora_ctx* c);
 int ora_timestep(ora_ctx* c);
 int ora_step(ora_ctx* c, int32_t nsteps);
+int ora_eddyvis_lagged(ora_ctx* c);
+int ora_scalart(ora_ctx* c);
+int ora_scalarupdate(ora_ctx* c, double stime);
+int ora_densityupdate(ora_ctx* c);
+int ora_salt_init(ora_ctx* c, double xl, double yl, double line_grad);
+int ora_get_substeps(ora_ctx* c, int32_t* solsteps, double* dtsol);
 
 /* Building blocks exposed for the oracle's own pinning tests. */
 double ora_kernel_w(const isph_params* prm, double r);

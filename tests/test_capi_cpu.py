@@ -36,7 +36,7 @@ def test_params_layout_matches_c(tmp_path):
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "isph_capi.h"\n'
                    'int main(void){printf("%zu %zu %zu %zu\\n", sizeof(isph_params),'
                    ' offsetof(isph_params, kernel), offsetof(isph_params, mu),'
-                   ' offsetof(isph_params, reserved));return 0;}\n')
+                   ' offsetof(isph_params, salinity));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
@@ -44,7 +44,7 @@ def test_params_layout_matches_c(tmp_path):
     assert size == ctypes.sizeof(IsphParams)
     assert off_kernel == IsphParams.kernel.offset
     assert off_mu == IsphParams.mu.offset
-    assert off_res == IsphParams.reserved.offset
+    assert off_res == IsphParams.salinity.offset
 
 
 def test_generators_sizes_and_determinism():
