@@ -99,6 +99,7 @@ def declare(lib, prefix):
         "scalarupdate": [P, ctypes.c_double],
         "salt_init": [P, ctypes.c_double, ctypes.c_double, ctypes.c_double],
         "get_substeps": [P, _c_ip, _c_dp],
+        "probe": [P, i32, _c_dp, _c_dp, _c_dp],
     }
     for name in SUBROUTINES:
         sig[name] = [P]
@@ -147,6 +148,14 @@ def load_library(path=LIB_PATH):
             ("adjoint_set", [ctypes.c_void_p, ctypes.c_int32, _c_dp], ctypes.c_int),
             ("adjoint_params", [ctypes.c_void_p, _c_dp, _c_dp], ctypes.c_int),
             ("objective_eval", [ctypes.c_void_p, ctypes.POINTER(IsphObjective), _c_dp],
+             ctypes.c_int),
+            ("combined", [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32], ctypes.c_int),
+            ("format_f", [ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p],
+             ctypes.c_int),
+            ("format_es", [ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p],
+             ctypes.c_int),
+            ("snapshot_record", [ctypes.c_double] * 6 + [ctypes.c_char_p], ctypes.c_int),
+            ("write_probe_series", [ctypes.c_char_p, ctypes.c_int32, _c_dp, _c_dp],
              ctypes.c_int),
         ]:
             f = getattr(lib, "isph_" + name)
@@ -335,6 +344,15 @@ class Sim:
         self._check(self._fn("salt_init")(self._h, float(xl), float(yl), float(line_grad)),
                     "salt_init")
 
+    def probe(self, points):
+        """Shepard pressure at (x, y) points (probe.f90:86-130)."""
+        pts = np.asarray(points, dtype=np.float64).reshape(-1, 2)
+        px = np.ascontiguousarray(pts[:, 0])
+        py = np.ascontiguousarray(pts[:, 1])
+        out = np.zeros(len(px))
+        self._check(self._fn("probe")(self._h, len(px), _dp(px), _dp(py), _dp(out)), "probe")
+        return out
+
     def substeps(self):
         """(solsteps, dtsol) of the next step's transport."""
         n, d = ctypes.c_int32(), ctypes.c_double()
@@ -426,6 +444,12 @@ class IsphSim(Sim):
                     "adjoint_params")
         return g, u.value
 
+    def combined(self, directory, index):
+        """Snapshot files fluid/fluid_<index>.txt, buffer/buffer_<index>.txt
+        and fluid/dead_<index>.txt under `directory` (output.f90:188-234)."""
+        self._check(self._lib.isph_combined(self._h, os.fsencode(directory), int(index)),
+                    "combined")
+
     def objective_eval(self, obj):
         J = ctypes.c_double()
         self._check(self._lib.isph_objective_eval(self._h, ctypes.byref(obj), ctypes.byref(J)),
@@ -455,3 +479,36 @@ def run_step_stages(sim, nsteps=1):
         sim.effpor()
         sim.opt2_shift()
         sim.timestep()
+
+
+def format_f(v, w, d):
+    """Fortran Fw.d of v, as the library writes it."""
+    buf = ctypes.create_string_buffer(w + 1)
+    if load_library().isph_format_f(float(v), w, d, buf) != 0:
+        raise ValueError("bad edit descriptor")
+    return buf.value.decode()
+
+
+def format_es(v, w, d):
+    """Fortran ESw.d of v, as the library writes it."""
+    buf = ctypes.create_string_buffer(w + 1)
+    if load_library().isph_format_es(float(v), w, d, buf) != 0:
+        raise ValueError("bad edit descriptor")
+    return buf.value.decode()
+
+
+def snapshot_record(x, y, p, vx, vy, con):
+    """One record of output.f90:206-209 (without the newline)."""
+    buf = ctypes.create_string_buffer(80)
+    load_library().isph_snapshot_record(x, y, p, vx, vy, con, buf)
+    return buf.value.decode()
+
+
+def write_probe_series(path, t, p, g=-9.81, wc=0.3, rho=1000.0):
+    """probe.txt of output.f90:150-166: records (t sqrt(|g|/wc),
+    p / (rho |g| wc)) (probe.f90:126-127) as (F10.3, 1X, F10.3)."""
+    a = np.ascontiguousarray(np.asarray(t, np.float64) * np.sqrt(abs(g) / wc))
+    b = np.ascontiguousarray(np.asarray(p, np.float64) / (rho * abs(g) * wc))
+    rc = load_library().isph_write_probe_series(os.fsencode(path), len(a), _dp(a), _dp(b))
+    if rc != 0:
+        raise OSError(f"cannot write {path}")

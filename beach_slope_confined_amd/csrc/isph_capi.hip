@@ -348,7 +348,7 @@ void prof_collect(isph_ctx* c) {
     Profiler& pr = c->prof;
     if (!pr.on) return;
     (void)hipStreamSynchronize(c->stream);
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < PROF_SLOTS; ++k) {
         for (size_t q = 0; q < pr.used[k]; ++q) {
             if (!pr.real[k][q]) continue;
             float ms = 0.f;
@@ -754,7 +754,7 @@ int isph_destroy(isph_ctx* c) {
         free_state(sl.v);
         if (sl.key) (void)hipFree(sl.key);
     }
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < PROF_SLOTS; ++k)
         for (auto& e : c->prof.pool[k]) {
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
@@ -1173,7 +1173,7 @@ int isph_profile_enable(isph_ctx* c, int32_t enable) {
     if (!c || enable < 0) return ISPH_E_ARG;
     c->prof.on = enable != 0;
     c->prof.stride = enable > 0 ? enable : 1;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < PROF_SLOTS; ++k) {
         c->prof.launches[k] = 0;
         c->prof.total_ms[k] = 0.0;
         c->prof.used[k] = 0;
@@ -1184,7 +1184,7 @@ int isph_profile_enable(isph_ctx* c, int32_t enable) {
 }
 
 int isph_get_kernel_time(isph_ctx* c, int32_t kernel, int64_t* launches, double* total_ms) {
-    if (!c || kernel < 0 || kernel > 3) return ISPH_E_ARG;
+    if (!c || kernel < 0 || kernel >= PROF_SLOTS) return ISPH_E_ARG;
     if (launches) *launches = c->prof.launches[kernel];
     if (total_ms) *total_ms = c->prof.total_ms[kernel];
     return ISPH_OK;

@@ -17,18 +17,21 @@ struct EventPair {
     hipEvent_t a, b;
 };
 
-// HIP events around every stride-th launch of the four SpMV kernel slots
+// HIP events around every stride-th launch of the timed kernel slots: the
+// four BiCGSTAB SpMV kernels (FP64 and mixed, p- and s-SpMV) and scalart
+constexpr int PROF_SLOTS = 5;
+constexpr int PROF_SCALART = 4;
 struct Profiler {
     bool on = false;
     int stride = 1;        // record every stride-th launch of a slot
-    int64_t seen[4] = {0, 0, 0, 0};
-    bool cur[4] = {false, false, false, false};
-    std::vector<EventPair> pool[4];
-    std::vector<char> real[4];      // per recorded launch: did real work
-    size_t used[4] = {0, 0, 0, 0};
-    int64_t launches[4] = {0, 0, 0, 0};
-    double total_ms[4] = {0, 0, 0, 0};
-    int batch_recorded[4] = {0, 0, 0, 0};
+    int64_t seen[PROF_SLOTS] = {};
+    bool cur[PROF_SLOTS] = {};
+    std::vector<EventPair> pool[PROF_SLOTS];
+    std::vector<char> real[PROF_SLOTS];      // per recorded launch: did real work
+    size_t used[PROF_SLOTS] = {};
+    int64_t launches[PROF_SLOTS] = {};
+    double total_ms[PROF_SLOTS] = {};
+    int batch_recorded[PROF_SLOTS] = {};
 };
 
 struct AdjointState;  // isph_adjoint.hip
@@ -69,7 +72,7 @@ struct isph_ctx {
     uint32_t* err = nullptr;
     double2* pos_nb = nullptr;     // positions at the last neighbour search
     double* cdiff = nullptr;       // salinity rate of the last scalart
-    double2* salt_coef = nullptr;  // per-particle dispersion (Dx, Dy)
+    double4* salt_coef = nullptr;  // per particle (Dx, Dy, Dm + nut/ts, nut)
     double2* spos = nullptr;
     double *smass = nullptr, *srho = nullptr;
     int* sstart = nullptr;
@@ -119,6 +122,7 @@ int take_errors(isph_ctx* c);
 int sync_check(isph_ctx* c);
 int read_sc(isph_ctx* c);
 void prof_collect(isph_ctx* c);
+void prof_record(isph_ctx* c, int k, bool start);
 
 // stages of the step, enqueue only (main.f90:41-66 order)
 int sort_particles(isph_ctx* c, bool with_sink);

@@ -275,10 +275,10 @@ struct SaltConsts {
     double rhomax, rhomin, xbuf;
 };
 __global__ void k_scalar_coef(int n, SaltConsts sc, StateView s, const SimScalars* ss,
-                              double2* coef);
+                              double4* coef);
 __global__ void k_scalart(int n, KernelConsts kh, SaltConsts sc, size_t ld, StateView s,
                           const double2* pos_nb, const int* nbr, const int* cnt,
-                          const double2* coef, double* cdiff);
+                          const double4* coef, double* cdiff);
 __global__ void k_scalarupdate(int n, SaltConsts sc, StateView s, const double* cdiff,
                                double stime, const SimScalars* ss);
 __global__ void k_densityupdate(int n, SaltConsts sc, StateView s);

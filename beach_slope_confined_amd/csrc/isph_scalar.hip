@@ -8,12 +8,12 @@
 //
 // Layout: con and the particle volume ovol travel with the particle (they
 // are members of StateView and follow every re-binning); cdiff and the
-// per-particle dispersion coefficients (Dx, Dy) are scratch in the current
-// order.  The coefficients depend only on the particle's own velocity,
-// porosity and dt, none of which change during the transport substeps, so
-// k_scalar_coef evaluates them once per step, in one pass over N, instead
-// of once per pair as scalar.f90:160-170 does; the values are the same
-// expression and the same bits.
+// per-particle coefficients (Dx, Dy, Dm + nut/ts, nut) are scratch in the
+// current order.  The coefficients depend only on the particle's own
+// velocity, porosity, eddy viscosity and dt, none of which change during
+// the transport substeps, so k_scalar_coef evaluates them once per step, in
+// one pass over N, instead of once per pair as scalar.f90:160-192 does; the
+// values are the same expressions and the same bits.
 //
 // scalart reads the lists as the reference stores them: pair distances r
 // from the positions of the last neighbour search (pos_nb) and position
@@ -53,44 +53,45 @@ __device__ inline double2 salt_dispersion(double vx, double vy, double n, double
 }
 
 __global__ void k_scalar_coef(int n, SaltConsts sc, StateView s, const SimScalars* ss,
-                              double2* __restrict__ coef) {
+                              double4* __restrict__ coef) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n || !salt_live(s, i)) return;
     const double cap = (0.12 * (sc.dl * sc.dl)) / ss->dt;
     const double2 v = s.vel[i];
-    coef[i] = salt_dispersion(v.x, v.y, s.por[i], cap);
+    const double2 d = salt_dispersion(v.x, v.y, s.por[i], cap);
+    const double nut = s.nut[i];
+    coef[i] = make_double4(d.x, d.y, DM_SALT + nut / sc.tschmidt, nut);
 }
 
 // scalart, scalar.f90:114-236: one thread per fluid particle, pull over its
 // list; only live fluid neighbours contribute (:158).
 __global__ void k_scalart(int n, KernelConsts kh, SaltConsts sc, size_t ld, StateView s,
                           const double2* __restrict__ pos_nb, const int* __restrict__ nbr,
-                          const int* __restrict__ cnt, const double2* __restrict__ coef,
+                          const int* __restrict__ cnt, const double4* __restrict__ coef,
                           double* __restrict__ cdiff) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n || !salt_live(s, i)) return;
     const double con1 = s.con[i];
-    const double2 di = coef[i];
-    const double ni = s.por[i], nuti = s.nut[i], oi = s.ovol[i];
-    const double ai = DM_SALT + nuti / sc.tschmidt;
+    const double4 di = coef[i];
+    const double ni = s.por[i], oi = s.ovol[i];
+    const double ai = di.z, nuti = di.w;
     const double2 pi = s.pos[i], qi = pos_nb[i];
     double acc = 0.0;
     const int c = cnt[i];
     for (int k = 0; k < c; ++k) {
         const int j = nbr[(size_t)k * ld + i];
         if (!salt_live(s, j)) continue;
-        const double nj = s.por[j], nutj = s.nut[j];
+        const double nj = s.por[j];
+        const double4 dj = coef[j];
         double t1x, t1y;
         if (ni >= 0.8 || nj >= 0.8) {  // open water: molecular + turbulent (:184-196)
-            if (nuti < TINY6 && nutj < TINY6) {
-                t1x = DM_SALT + ((nuti + nutj) * 0.50) / sc.tschmidt;
+            if (nuti < TINY6 && dj.w < TINY6) {
+                t1x = DM_SALT + ((nuti + dj.w) * 0.50) / sc.tschmidt;
             } else {
-                const double aj = DM_SALT + nutj / sc.tschmidt;
-                t1x = 2.0 * ((ai * aj) / (ai + aj));
+                t1x = 2.0 * ((ai * dj.z) / (ai + dj.z));
             }
             t1y = t1x;
         } else {  // in the pores: harmonic mean of the dispersion (:200-201)
-            const double2 dj = coef[j];
             t1x = (((2.0 * TOR_SALT) * di.x) * dj.x) / (di.x + dj.x);
             t1y = (((2.0 * TOR_SALT) * di.y) * dj.y) / (di.y + dj.y);
         }
@@ -180,10 +181,13 @@ int run_scalar_coef(isph_ctx* c) {
 // cdiff from the coefficients of the last run_scalar_coef
 int run_scalart_only(isph_ctx* c) {
     if (!c->have_nbr) return fail(c, ISPH_E_STATE, "scalart before neighbour");
-    if (c->n)
+    if (c->n) {
+        prof_record(c, PROF_SCALART, true);
         k_scalart<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, hybrid_kc(c), salt_consts(c),
                                                         (size_t)c->cap, c->st[c->cur], c->pos_nb,
                                                         c->nbr, c->cnt, c->salt_coef, c->cdiff);
+        prof_record(c, PROF_SCALART, false);
+    }
     HIPCHK(c, hipGetLastError());
     return ISPH_OK;
 }
@@ -243,6 +247,7 @@ extern "C" {
 int isph_scalart(isph_ctx* c) {
     if (!c) return ISPH_E_ARG;
     int rc = run_scalart(c);
+    prof_collect(c);
     if (rc) return rc;
     return sync_check(c);
 }

@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-salinity", action="store_true")
+    ap.add_argument("--salinity-only", action="store_true",
+                    help="run only the salt-water step leg (for profiling)")
+    ap.add_argument("--salt-dx", type=float, default=0.001)
     ap.add_argument("--adjoint-dx", type=float, default=0.002)
     ap.add_argument("--adjoint-steps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=400,
@@ -108,6 +112,94 @@ def run_gpu(cfg, steps, warmup, precision, profile):
         k0, k1 = (0, 1) if precision == PREC_FP64 else (2, 3)
         out["ktime"] = {"p_spmv": sim.kernel_time(k0), "s_spmv": sim.kernel_time(k1)}
     return out
+
+
+def scalart_bytes(ptr, tid):
+    """Algorithmic bytes of one scalart launch: per live fluid row its own
+    state (tid, flags, con, (Dx, Dy), n, nut, ovol, position, search
+    position, cnt: 96 B), its list (4 B per neighbour) and the cdiff write
+    (8 B); of the other particles tid and flags (8 B).  Neighbour data is
+    counted once, as the particle's own."""
+    cnt = np.diff(ptr)
+    fl = tid == 3
+    return int(104 * fl.sum() + 4 * cnt[fl].sum() + 8 * (~fl).sum())
+
+
+def run_salinity(dx, steps, warmup):
+    """The salt-water step (main.f90:199-263) on the porous dam break at
+    spacing dx: salt water above the line y - 0.1 = 0.3 (x - 0.2) in the
+    column, fresh below; transport substeps on the previous step's lists,
+    densityupdate, the lagged eddyvis.  Steps run back to back (the
+    transport reads the lists of the previous step)."""
+    import torch
+    from beach_slope_confined_amd import capi, configs
+    cfg = configs.dam_break(dx)
+    cfg["params"] = cfg["params"].copy(salinity=1)
+    sim = configs.make_sim(cfg, capi.IsphSim)
+    sim.salt_init(0.2, 0.1, 0.3)
+    for _ in range(max(warmup, 1)):   # the first step builds the lists
+        sim.step(1)
+    sim.profile_enable(1)
+    sub = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sub.append(sim.substeps()[0])
+        sim.step(1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    iters = sim.scalars()["iters"]
+    launches, tot = sim.kernel_time(4)
+    ptr, _ = sim.get_neighbours()
+    tid = sim.get_particles()["tid"]
+    nb = scalart_bytes(ptr, tid)
+    avg_s = tot / max(launches, 1) / 1e3
+    gbs = nb / avg_s / 1e9 if launches else 0.0
+    con = sim.get_field("con")[tid == 3]
+    sim.close()
+    n_fluid = cfg["n_fluid"]
+    n = len(cfg["parts"]["x"])
+    return {"workload": f"dam_break dx={dx:g}: {n_fluid} fluid, {n} particles, "
+                        f"{len(cfg['soil']['x'])} soil; salt-water step, salt above "
+                        f"y - 0.1 = 0.3 (x - 0.2)",
+            "ppe_iters_last_step": iters,
+            "value": round(n_fluid * steps / (t1 - t0), 1),
+            "ms_per_step": round(1e3 * (t1 - t0) / steps, 3), "substeps_per_step": sub,
+            "con_range": [float(con.min()), float(con.max())],
+            "k_scalart": {"launches": launches, "avg_us": round(avg_s * 1e6, 2),
+                          "bytes_per_launch": nb, "achieved_GBs": round(gbs, 1),
+                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}}
+
+
+def run_scalart_at_scale(cfg, reps=20):
+    """scalart alone on the bench tank (1,057,810 particles), with pores in
+    the lower half, eddy viscosity on both sides of the 1e-6 threshold and
+    a rough salinity field, on the lists of one neighbour search: the
+    transport kernel at the bench's particle count (the open tank itself
+    cannot run two salt-water steps in a row)."""
+    from beach_slope_confined_amd import capi
+    n = len(cfg["parts"]["x"])
+    sim = capi.IsphSim(cfg["params"].copy(salinity=1), n, 1)
+    sim.set_particles(cfg["parts"])
+    sim.neighbour()
+    q = sim.get_particles()
+    x, y = q["x"], q["y"]
+    sim.set_field("porosity", np.where(y < 0.5, 0.4 + 0.2 * x, 1.0))
+    sim.set_field("nut", np.where(np.sin(9 * x) > 0, 3e-6, 4e-7) * (x > 0.3))
+    sim.set_field("con", np.clip(y - 0.5 + 0.1 * np.sin(30 * x), -0.5, 0.5))
+    sim.scalart()
+    sim.profile_enable(1)
+    for _ in range(reps):
+        sim.scalart()
+    launches, tot = sim.kernel_time(4)
+    ptr, _ = sim.get_neighbours()
+    nb = scalart_bytes(ptr, q["tid"])
+    sim.close()
+    avg_s = tot / max(launches, 1) / 1e3
+    gbs = nb / avg_s / 1e9 if launches else 0.0
+    return {"particles": n, "pairs": int(ptr[-1]), "launches": launches,
+            "avg_us": round(avg_s * 1e6, 2), "bytes_per_launch": nb,
+            "achieved_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def run_adjoint(dx, nsteps, every):
@@ -218,6 +310,11 @@ def main():
         return
     n_fluid = cfg["n_fluid"]
     n_total = len(cfg["parts"]["x"])
+    if args.salinity_only:
+        sal = run_salinity(args.salt_dx, args.steps, args.warmup)
+        sal["k_scalart_tank"] = run_scalart_at_scale(cfg)
+        print(json.dumps({"salinity": sal}), flush=True)
+        return
 
     r64 = run_gpu(cfg, args.steps, args.warmup, PREC_FP64, True)
     ms = 1e3 * r64["seconds"] / args.steps
@@ -293,6 +390,13 @@ def main():
                                     "achieved_GBs": round(bm / avgm / 1e9, 1) if lm else None},
             }
             rmx["sim"].close()
+
+    if not args.no_salinity:
+        try:
+            line["salinity"] = run_salinity(args.salt_dx, args.steps, args.warmup)
+            line["salinity"]["k_scalart_tank"] = run_scalart_at_scale(cfg)
+        except Exception as e:  # reported, not hidden
+            line["salinity"] = {"error": str(e)}
 
     if not args.no_adjoint:
         try:

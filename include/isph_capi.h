@@ -244,6 +244,34 @@ int isph_salt_init(isph_ctx* ctx, double xl, double yl, double line_grad);
  * first timestep, as the reference's solsteps = 0 (initialize.f90:56). */
 int isph_get_substeps(isph_ctx* ctx, int32_t* solsteps, double* dtsol);
 
+/* ---- text output and probes (output.f90, probe.f90) --------------------- */
+/* Fortran edit descriptors Fw.d and ESw.d as a Fortran run-time library
+ * prints them (correct rounding, ties to even; w asterisks when the value
+ * does not fit; ESw.d exponents E+dd, or +ddd without the E from 100 to 999;
+ * Inf, -Inf, NaN right-justified).  out receives w characters and a NUL. */
+int isph_format_f(double v, int32_t w, int32_t d, char* out);
+int isph_format_es(double v, int32_t w, int32_t d, char* out);
+/* One snapshot record, output.f90:206-209:
+ * (F10.3,1X,F10.3,1X,F10.3,1X,ES10.3,1X,ES10.3,1X,F10.5) of
+ * x, y, p, vx, vy, con + 0.5; out receives 65 characters and a NUL. */
+int isph_snapshot_record(double x, double y, double p, double vx, double vy,
+                         double con, char* out);
+/* combined (output.f90:188-234): writes <dir>/fluid/fluid_<index>.txt (live
+ * fluid), <dir>/buffer/buffer_<index>.txt (live buffer fluid) and
+ * <dir>/fluid/dead_<index>.txt (dead non-buffer fluid), one record per
+ * particle in the context's cell order.  The folders must exist. */
+int isph_combined(isph_ctx* ctx, const char* dir, int32_t index);
+/* probevalue (probe.f90:86-130): Shepard pressure at np points,
+ * sum_j p_j m_j W_j / rho_j over sum_j m_j W_j / rho_j for fluid j within
+ * 2h in the 3x3 cells around the point's cell; 0 where no fluid is in range. */
+int isph_probe(isph_ctx* ctx, int32_t np, const double* px, const double* py,
+               double* out);
+/* The probe series file of output.f90:150-166: n records (F10.3,1X,F10.3)
+ * of a[i], b[i] (the reference writes t sqrt(|g|/wc) and
+ * p / (rho |g| wc), probe.f90:126-127). */
+int isph_write_probe_series(const char* path, int32_t n, const double* a,
+                            const double* b);
+
 /* ---- in-memory checkpoints (no reference counterpart: the reference has no
  * restart, output.f90 only writes text snapshots) ---------------------------- */
 /* Saves the particle state, the cell order and the scalars (umax, dt, t) to
@@ -256,11 +284,11 @@ int isph_restore(isph_ctx* ctx, int32_t slot);
 
 /* ---- instrumentation (no reference counterpart) -------------------------- */
 /* enable = s > 0: HIP events bracket every s-th launch of each BiCGSTAB SpMV
- * kernel on the context stream (0 turns it off); isph_get_kernel_time()
- * returns the timed launches and their total milliseconds since the last
- * enable.  kernel: 0 = FP64 step-1 SpMV (p update, v = A p), 1 = FP64
- * step-2 SpMV (s, t = A s), 2 and 3 = the same for the mixed-precision
- * flavours. */
+ * kernel and of the scalart kernel on the context stream (0 turns it off);
+ * isph_get_kernel_time() returns the timed launches and their total
+ * milliseconds since the last enable.  kernel: 0 = FP64 step-1 SpMV
+ * (p update, v = A p), 1 = FP64 step-2 SpMV (s, t = A s), 2 and 3 = the
+ * same for the mixed-precision flavours, 4 = scalart. */
 int isph_profile_enable(isph_ctx* ctx, int32_t enable);
 int isph_get_kernel_time(isph_ctx* ctx, int32_t kernel, int64_t* launches,
                          double* total_ms);

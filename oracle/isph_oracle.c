@@ -1635,6 +1635,40 @@ int ora_salt_init(ora_ctx* c, double xl, double yl, double line_grad) {
     return ISPH_OK;
 }
 
+/* probe.f90:86-130 probevalue: Shepard pressure at each point over fluid
+ * (tid > 2) within 2 blen h (blen = 1) in the point's cell and its eight
+ * neighbours, visited in the order of probe.f90:92-94 ((i,j), (i,j-1),
+ * (i,j+1), (i+1,j), (i-1,j), (i+1,j+1), (i+1,j-1), (i-1,j-1), (i-1,j+1)
+ * with i the row (y) and j the column (x) of the cell). */
+int ora_probe(ora_ctx* c, int32_t np, const double* px, const double* py, double* out) {
+    static const int oy[9] = {0, 0, 0, 1, -1, 1, 1, -1, -1};
+    static const int ox[9] = {0, -1, 1, 0, 0, 1, -1, -1, 1};
+    const double h2x = 2.0 * c->prm.h;
+    int oob = 0;
+    for (int32_t k = 0; k < np; ++k) {
+        int32_t cell = cell_of(c, px[k], py[k], &oob);
+        int32_t cx = cell / c->ny, cy = cell % c->ny;
+        double term1 = 0.0, pr = 0.0;
+        int32_t count = 0;
+        for (int q = 0; q < 9; ++q) {
+            int32_t cc = (cx + ox[q]) * c->ny + (cy + oy[q]);
+            for (int32_t j = c->bstart[2 * cc]; j < c->bstart[2 * cc + 2]; ++j) {
+                if (c->tid[j] <= 2) continue;
+                double dx = px[k] - c->x[j], dy = py[k] - c->y[j];
+                double r = sqrt(dx * dx + dy * dy);
+                if (!(r < h2x)) continue;
+                double w = ora_kernel_w(&c->prm, r);
+                term1 = term1 + (c->mass[j] * w) / c->rho[j];
+                pr = pr + ((c->p[j] * c->mass[j]) * w) / c->rho[j];
+                count++;
+            }
+        }
+        out[k] = count ? pr / term1 : 0.0;
+    }
+    if (oob) return set_err(c, ISPH_E_DOMAIN, "probe outside the cell grid");
+    return ISPH_OK;
+}
+
 int ora_get_substeps(ora_ctx* c, int32_t* solsteps, double* dtsol) {
     if (solsteps) *solsteps = c->solsteps;
     if (dtsol) *dtsol = c->dtsol;

@@ -72,6 +72,7 @@ int ora_scalarupdate(ora_ctx* c, double stime);
 int ora_densityupdate(ora_ctx* c);
 int ora_salt_init(ora_ctx* c, double xl, double yl, double line_grad);
 int ora_get_substeps(ora_ctx* c, int32_t* solsteps, double* dtsol);
+int ora_probe(ora_ctx* c, int32_t np, const double* px, const double* py, double* out);
 
 /* Building blocks exposed for the oracle's own pinning tests. */
 double ora_kernel_w(const isph_params* prm, double r);
