@@ -175,6 +175,20 @@ def tank_1m(nside=1024, seed=SEED, jitter=0.1):
     return dict(name=f"tank_{nside}x{nside}", parts=parts, soil=soil, params=prm, n_fluid=nf)
 
 
+def with_reference_cells(cfg):
+    """The config on the reference's cell grid and particle order: cells of
+    4h (incr * h1, setup.f90:120-123) with boundaries at multiples of 4h,
+    the lowest cells starting at -4h as the reference's j = 1 and i = celly,
+    and cell_order = ORDER_REFERENCE."""
+    from .params import ORDER_REFERENCE
+    p = cfg["params"]
+    cs = 4.0 * p.h
+    out = dict(cfg)
+    out["params"] = p.copy(cell_size=cs, xmin=-cs, ymin=-cs, cell_order=ORDER_REFERENCE)
+    out["name"] = cfg["name"] + "_refcells"
+    return out
+
+
 def make_sim(cfg, sim_cls, cap_extra=0, **kw):
     """Create a context of `sim_cls` loaded with a config and initialised
     (effpor + timestep, main.f90:35-37)."""

@@ -270,7 +270,7 @@ int tape_step(isph_ctx* c, StepTape& t) {
     if ((rc = copy_state(c, c->st[c->cur], t.Sa, c->n))) return rc;
     k_advance_time<<<1, 64, 0, c->stream>>>(c->ss);
     if ((rc = run_projection(c))) return rc;
-    if ((rc = sort_particles(c, false))) return rc;
+    if ((rc = sort_particles(c, false, false))) return rc;
     if ((rc = copy_dev(c, t.perm1, c->perm, sizeof(int) * c->n))) return rc;
     if ((rc = copy_dev(c, t.rho0, c->st[c->cur].rho, sizeof(double) * c->n))) return rc;
     if ((rc = run_neighbour(c))) return rc;
@@ -289,7 +289,7 @@ int tape_step(isph_ctx* c, StepTape& t) {
     if ((rc = run_comp_vel(c))) return rc;
     if ((rc = copy_dev(c, t.vel1, c->st[c->cur].vel, sizeof(double2) * c->n))) return rc;
     if ((rc = run_comp_pos(c))) return rc;
-    if ((rc = sort_particles(c, true))) return rc;
+    if ((rc = sort_particles(c, true, false))) return rc;
     if ((rc = copy_dev(c, t.perm2, c->perm, sizeof(int) * c->n))) return rc;
     if ((rc = copy_dev(c, t.rho1p, c->st[c->cur].rho, sizeof(double) * c->n))) return rc;
     if ((rc = run_neighbour(c))) return rc;
@@ -474,6 +474,8 @@ int check_adjoint_params(isph_ctx* c) {
     if (c->prm.sink) return fail(c, ISPH_E_ARG, "the adjoint does not support the outflow sink");
     if (c->prm.salinity)
         return fail(c, ISPH_E_ARG, "the adjoint does not support salinity transport (salinity = 0)");
+    if (c->prm.cell_order != ISPH_ORDER_CELL)
+        return fail(c, ISPH_E_ARG, "the adjoint runs with cell_order = ISPH_ORDER_CELL");
     return ISPH_OK;
 }
 

@@ -37,7 +37,9 @@ GridInfo make_grid(const isph_params& p) {
     g.cs = p.cell_size;
     g.nx = (int)std::floor((p.xmax - p.xmin) / p.cell_size) + 3;
     g.ny = (int)std::floor((p.ymax - p.ymin) / p.cell_size) + 3;
-    g.nbins = 2LL * g.nx * g.ny;
+    g.order = p.cell_order;
+    g.per = p.cell_order ? 10 : 2;
+    g.nbins = (long long)g.per * g.nx * g.ny;
     return g;
 }
 
@@ -45,8 +47,12 @@ int check_params(isph_ctx* c, const isph_params* p) {
     if (!(p->h > 0.0) || !(p->cell_size >= 2.0 * p->h) || !(p->xmax > p->xmin) ||
         !(p->ymax > p->ymin) || p->max_nbr <= 0)
         return fail(c, ISPH_E_ARG, "invalid geometry parameters");
+    if (p->cell_order != ISPH_ORDER_CELL && p->cell_order != ISPH_ORDER_REFERENCE)
+        return fail(c, ISPH_E_ARG, "unknown cell_order");
+    if (p->cell_order == ISPH_ORDER_REFERENCE && !(p->cell_size >= 4.0 * p->h))
+        return fail(c, ISPH_E_ARG, "the reference order needs cell_size >= 4h (2x2-cell blocks)");
     GridInfo g = make_grid(*p);
-    if ((double)g.nx * (double)g.ny * 2.0 > 2.0e9) return fail(c, ISPH_E_ARG, "cell grid too large");
+    if ((double)g.nx * (double)g.ny * g.per > 2.0e9) return fail(c, ISPH_E_ARG, "cell grid too large");
     return ISPH_OK;
 }
 
@@ -116,8 +122,9 @@ int scan(isph_ctx* c, int* data, int64_t count) {
     return ISPH_OK;
 }
 
-// stable counting sort by key = 2*cell + fluid bit (deleted: key nbins)
-int sort_particles(isph_ctx* c, bool with_sink) {
+// stable counting sort by bin key (deleted: key nbins); `first`: no
+// previous binning (isph_set_particles)
+int sort_particles(isph_ctx* c, bool with_sink, bool first) {
     const int n = c->n;
     const int64_t nb = c->grid.nbins;
     StateView& s = c->st[c->cur];
@@ -126,7 +133,7 @@ int sort_particles(isph_ctx* c, bool with_sink) {
     if (n > 0)
         k_bin_keys<<<nblk(n), BLOCK, 0, c->stream>>>(
             n, c->grid, s.pos, s.tid, s.flags, with_sink && c->prm.sink, c->prm.xlcutoff,
-            c->prm.ytcutoff, c->key, c->bincnt, c->rank, c->err);
+            c->prm.ytcutoff, first ? nullptr : c->key_s, c->key, c->bincnt, c->rank, c->err);
     int rc = scan(c, c->bincnt, nb + 2);
     if (rc) return rc;
     if (n > 0) {
@@ -186,8 +193,8 @@ int run_effpor(isph_ctx* c) {
     if (c->n)
         k_effpor<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->grid, kconsts(c),
                                                        porous_params(c->prm), c->st[c->cur],
-                                                       c->spos, c->smass, c->srho, c->sstart,
-                                                       c->err);
+                                                       c->key_s, c->spos, c->smass, c->srho,
+                                                       c->sstart, c->err);
     HIPCHK(c, hipGetLastError());
     return ISPH_OK;
 }
@@ -608,7 +615,7 @@ int run_step(isph_ctx* c) {
     k_advance_time<<<1, 64, 0, c->stream>>>(c->ss);
     if (salt && (rc = run_salt_substeps(c))) return rc;
     if ((rc = run_projection(c))) return rc;
-    if ((rc = sort_particles(c, false))) return rc;
+    if ((rc = sort_particles(c, false, false))) return rc;
     if ((rc = run_neighbour(c))) return rc;
     if ((rc = run_effpor(c))) return rc;
     if ((rc = run_eddyvis(c))) return rc;
@@ -618,7 +625,7 @@ int run_step(isph_ctx* c) {
     if ((rc = take_errors(c))) return rc;
     if ((rc = run_comp_vel(c))) return rc;
     if ((rc = run_comp_pos(c))) return rc;
-    if ((rc = sort_particles(c, true))) return rc;
+    if ((rc = sort_particles(c, true, false))) return rc;
     if ((rc = run_neighbour(c))) return rc;
     if ((rc = run_freesurf(c))) return rc;
     if ((rc = run_effpor(c))) return rc;
@@ -707,7 +714,7 @@ int isph_create(const isph_params* prm, int32_t cap, int32_t cap_soil, isph_ctx*
     rc |= dalloc(c, &c->spos, (size_t)std::max(cap_soil, 1));
     rc |= dalloc(c, &c->smass, (size_t)std::max(cap_soil, 1));
     rc |= dalloc(c, &c->srho, (size_t)std::max(cap_soil, 1));
-    rc |= dalloc(c, &c->sstart, (size_t)(nb / 2) + 1);
+    rc |= dalloc(c, &c->sstart, (size_t)c->grid.nx * c->grid.ny + 1);
     rc |= dalloc(c, &c->pos_nb, N);
     rc |= dalloc(c, &c->cdiff, N);
     rc |= dalloc(c, &c->salt_coef, N);
@@ -831,7 +838,7 @@ int isph_set_particles(isph_ctx* c, int32_t n, const double* x, const double* y,
     SimScalars ss0 = {0.0, 0.0, c->prm.dt, 0.0, c->prm.dt, 0, 0};
     HIPCHK(c, hipMemcpyAsync(c->ss, &ss0, sizeof(ss0), H2D, c->stream));
     c->n = n;
-    int rc = sort_particles(c, false);
+    int rc = sort_particles(c, false, true);
     if (rc) return rc;
     return sync_check(c);
 }
@@ -877,7 +884,7 @@ int isph_set_soil(isph_ctx* c, int32_t ns, const double* x, const double* y, con
     if (!c) return ISPH_E_ARG;
     if (ns < 0 || ns > c->cap_soil) return fail(c, ISPH_E_ARG, "ns exceeds soil capacity");
     const GridInfo& g = c->grid;
-    const int64_t nc = g.nbins / 2;
+    const int64_t nc = (int64_t)g.nx * g.ny;
     std::vector<int> cnt(nc + 1, 0), k(ns);
     bool oob = false;
     for (int i = 0; i < ns; ++i) {
@@ -1086,8 +1093,8 @@ int isph_set_scalars(isph_ctx* c, double umax, double dt, double t) {
         return sync_check(c);               \
     }
 
-static int cellshift_(isph_ctx* c) { return sort_particles(c, false); }
-static int cellshiftalt_(isph_ctx* c) { return sort_particles(c, true); }
+static int cellshift_(isph_ctx* c) { return sort_particles(c, false, false); }
+static int cellshiftalt_(isph_ctx* c) { return sort_particles(c, true, false); }
 static int resetid_(isph_ctx* c) { (void)c; return ISPH_OK; }
 
 STAGE(isph_projection, run_projection)

@@ -125,7 +125,7 @@ void prof_collect(isph_ctx* c);
 void prof_record(isph_ctx* c, int k, bool start);
 
 // stages of the step, enqueue only (main.f90:41-66 order)
-int sort_particles(isph_ctx* c, bool with_sink);
+int sort_particles(isph_ctx* c, bool with_sink, bool first);
 int run_projection(isph_ctx* c);
 int run_neighbour(isph_ctx* c);
 int run_effpor(isph_ctx* c);

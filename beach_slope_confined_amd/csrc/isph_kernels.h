@@ -34,8 +34,53 @@ constexpr int DONE_UPDATE = 5;      // mixed precision: reliable update requeste
 struct GridInfo {
     double xmin, ymin, cs;
     int nx, ny;
-    long long nbins;  // 2 * nx * ny
+    long long nbins;  // per * nx * ny
+    int order;        // ISPH_ORDER_*
+    int per;          // bins per cell: 2 (walls, fluid) or 10 (walls, stayers, 8 arrival sides)
 };
+
+// Bin layout.  ISPH_ORDER_CELL: bin = 2 * (cx*ny + cy) + fluid bit.
+// ISPH_ORDER_REFERENCE: cells ranked column by column with y descending (the
+// reference's resetid numbering, isph.f90:96-109, row i = 1 at the top), bin
+// = 10 * rank + class, class 0 walls, 1 fluid that stayed, 2..9 arrivals in
+// the gather order of integrator.f90:318-389.
+__host__ __device__ inline int cell_rank(const GridInfo& g, int cx, int cy) {
+    return g.order ? cx * g.ny + (g.ny - 1 - cy) : cx * g.ny + cy;
+}
+
+// cell of a bin key
+__host__ __device__ inline void key_cell(const GridInfo& g, int key, int& cx, int& cy) {
+    if (g.order) {
+        const int r = key / 10;
+        cx = r / g.ny;
+        cy = g.ny - 1 - (r - cx * g.ny);
+    } else {
+        const int c = key >> 1;
+        cx = c / g.ny;
+        cy = c - cx * g.ny;
+    }
+}
+
+// bins [lo, hi) of the cells ylo..yhi of column cx (contiguous in both layouts)
+__host__ __device__ inline void column_bins(const GridInfo& g, int cx, int ylo, int yhi, int& lo,
+                                            int& hi) {
+    if (g.order) {
+        lo = 10 * (cx * g.ny + (g.ny - 1 - yhi));
+        hi = 10 * (cx * g.ny + (g.ny - 1 - ylo)) + 10;
+    } else {
+        lo = 2 * (cx * g.ny + ylo);
+        hi = 2 * (cx * g.ny + yhi) + 2;
+    }
+}
+
+// class of a fluid particle that moved by (dcx, dcy) in {-1,0,1}^2 cells
+// (y up): stay 1; from below 2 (tn), above 3 (ts), right 4 (tw), left 5 (te),
+// upper left 6 (tse), lower left 7 (tne), upper right 8 (tsw), lower right 9 (tnw)
+__host__ __device__ inline int move_class(int dcx, int dcy) {
+    // [dcy + 1][dcx + 1]
+    const int t[3][3] = {{8, 3, 6}, {4, 1, 5}, {9, 2, 7}};
+    return t[dcy + 1][dcx + 1];
+}
 
 struct PorousParams {
     double por, alpha, beta, d50, nu;
@@ -209,8 +254,8 @@ constexpr int FIN_START = 6;    // (rhat,r) of the mixed start
 constexpr int FIN_REPLACE = 7;  // (r,rhat) after a reliable update
 
 __global__ void k_bin_keys(int n, GridInfo g, const double2* pos, const int* tid, const int* flags,
-                           int sink, double xlcut, double ytcut, int* key, int* bincnt, int* rank,
-                           uint32_t* err);
+                           int sink, double xlcut, double ytcut, const int* prev_key, int* key,
+                           int* bincnt, int* rank, uint32_t* err);
 __global__ void k_scan_blocks(int n, int* data, int* bsum);
 __global__ void k_scan_sums(int nb, int* bsum);
 __global__ void k_scan_add(int n, int* data, const int* bsum);
@@ -223,7 +268,7 @@ __global__ void k_neighbour(int n, GridInfo g, KernelConsts kc, double thr, int 
                             StateView s, const int* key, const int* bstart, int* nbr, int* cnt,
                             double* L, double* divr, uint32_t* err);
 __global__ void k_effpor(int n, GridInfo g, KernelConsts kc, PorousParams pp, StateView s,
-                         const double2* spos, const double* sm, const double* sr,
+                         const int* key, const double2* spos, const double* sm, const double* sr,
                          const int* sstart, uint32_t* err);
 __global__ void k_eddyvis(int n, KernelConsts kc, double dl, size_t ld, StateView s, const int* nbr,
                           const int* cnt, const double* L);

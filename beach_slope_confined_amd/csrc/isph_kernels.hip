@@ -15,14 +15,17 @@ namespace isph {
 __device__ __forceinline__ void raise_err(uint32_t* err, uint32_t bit) { atomicOr(err, bit); }
 
 // ---------------------------------------------------------------------------
-// cell binning: stable counting sort by key = 2*cell + fluid bit
-// (cellshift/cellshiftalt re-binning, integrator.f90:32-391, + resetid,
-// isph.f90:88-113)
+// cell binning: stable counting sort by bin key (cellshift/cellshiftalt
+// re-binning, integrator.f90:32-391, + resetid, isph.f90:88-113).  With the
+// reference order a fluid particle moves at most one cell from its previous
+// bin (prev_key; null on the first binning), as cellshift only hands a
+// particle to the adjacent cell it crossed into (integrator.f90:271-310).
 // ---------------------------------------------------------------------------
 __global__ void k_bin_keys(int n, GridInfo g, const double2* __restrict__ pos,
                            const int* __restrict__ tid, const int* __restrict__ flags, int sink,
-                           double xlcut, double ytcut, int* __restrict__ key,
-                           int* __restrict__ bincnt, int* __restrict__ rank, uint32_t* err) {
+                           double xlcut, double ytcut, const int* __restrict__ prev_key,
+                           int* __restrict__ key, int* __restrict__ bincnt,
+                           int* __restrict__ rank, uint32_t* err) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     double2 p = pos[i];
@@ -32,8 +35,22 @@ __global__ void k_bin_keys(int n, GridInfo g, const double2* __restrict__ pos,
         k = (int)g.nbins;
     } else {
         bool oob = false;
-        k = 2 * cell_of(g, p.x, p.y, oob) + fl;
+        const int c = cell_of(g, p.x, p.y, oob);
         if (oob) raise_err(err, ERR_DOMAIN);
+        if (!g.order) {
+            k = 2 * c + fl;
+        } else {
+            int cx = c / g.ny, cy = c - cx * g.ny, cls = fl;
+            if (fl && prev_key) {
+                int ox, oy;
+                key_cell(g, prev_key[i], ox, oy);
+                const int dx = max(-1, min(1, cx - ox)), dy = max(-1, min(1, cy - oy));
+                cx = ox + dx;
+                cy = oy + dy;
+                cls = move_class(dx, dy);
+            }
+            k = 10 * cell_rank(g, cx, cy) + cls;
+        }
     }
     key[i] = k;
     rank[i] = atomicAdd(&bincnt[k], 1);
@@ -164,14 +181,15 @@ __global__ void __launch_bounds__(BLOCK) k_neighbour(int n, GridInfo g, KernelCo
                                                      double* __restrict__ divr, uint32_t* err) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    const int cell = key[i] >> 1;
-    const int cx = cell / g.ny, cy = cell - cx * g.ny;
+    int cx, cy;
+    key_cell(g, key[i], cx, cy);
     const double2 pi = s.pos[i];
     int c = 0;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     for (int dc = -1; dc <= 1; ++dc) {
-        const int col = (cx + dc) * g.ny + cy;
-        const int lo = bstart[2 * (col - 1)], hi = bstart[2 * (col + 1) + 2];
+        int blo, bhi;
+        column_bins(g, cx + dc, cy - 1, cy + 1, blo, bhi);
+        const int lo = bstart[blo], hi = bstart[bhi];
         for (int j = lo; j < hi; ++j) {
             if (j == i) continue;
             const double2 pj = s.pos[j];
@@ -216,10 +234,23 @@ __global__ void __launch_bounds__(BLOCK) k_neighbour(int n, GridInfo g, KernelCo
 // ---------------------------------------------------------------------------
 // effpor, porous.f90:107-224 (+ porsearch :32-105)
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double soil_cells(const KernelConsts& kc, double cpor, double2 pi,
+                                             int lo, int hi, const double2* __restrict__ spos,
+                                             const double* __restrict__ sm,
+                                             const double* __restrict__ sr, double sf) {
+    for (int q = lo; q < hi; ++q) {
+        const double2 ps = spos[q];
+        double dx = pi.x - ps.x, dy = pi.y - ps.y;
+        double r = sqrt(dx * dx + dy * dy);
+        if (r < kc.twoh) sf = sf + ((cpor * kern_w(kc, r)) * sm[q]) / sr[q];
+    }
+    return sf;
+}
+
 __global__ void k_effpor(int n, GridInfo g, KernelConsts kc, PorousParams pp, StateView s,
-                         const double2* __restrict__ spos, const double* __restrict__ sm,
-                         const double* __restrict__ sr, const int* __restrict__ sstart,
-                         uint32_t* err) {
+                         const int* __restrict__ key, const double2* __restrict__ spos,
+                         const double* __restrict__ sm, const double* __restrict__ sr,
+                         const int* __restrict__ sstart, uint32_t* err) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     s.R[i] = 0.0;
@@ -228,17 +259,32 @@ __global__ void k_effpor(int n, GridInfo g, KernelConsts kc, PorousParams pp, St
     bool oob = false;
     const int cell = cell_of(g, pi.x, pi.y, oob);
     if (oob) raise_err(err, ERR_DOMAIN);
-    const int cx = cell / g.ny, cy = cell - cx * g.ny;
     double sf = 0.0;
     const double cpor = 1.0 - pp.por * 1.0;
-    for (int dc = -1; dc <= 1; ++dc) {
-        const int col = (cx + dc) * g.ny + cy;
-        const int lo = sstart[col - 1], hi = sstart[col + 2];
-        for (int q = lo; q < hi; ++q) {
-            const double2 ps = spos[q];
-            double dx = pi.x - ps.x, dy = pi.y - ps.y;
-            double r = sqrt(dx * dx + dy * dy);
-            if (r < kc.twoh) sf = sf + ((cpor * kern_w(kc, r)) * sm[q]) / sr[q];
+    if (!g.order) {
+        const int cx = cell / g.ny, cy = cell - cx * g.ny;
+        for (int dc = -1; dc <= 1; ++dc) {
+            const int col = (cx + dc) * g.ny + cy;
+            sf = soil_cells(kc, cpor, pi, sstart[col - 1], sstart[col + 2], spos, sm, sr, sf);
+        }
+    } else {
+        // porous.f90:127-176: the particle's own cell (its bin), then three
+        // cells of the 2x2 block on the side of the cell's midlines it lies on
+        int cx, cy;
+        key_cell(g, key[i], cx, cy);
+        const double xl = g.xmin + (double)(cx - 1) * g.cs, xr = g.xmin + (double)cx * g.cs;
+        const double yb = g.ymin + (double)(cy - 1) * g.cs, yt = g.ymin + (double)cy * g.cs;
+        const bool left = pi.x <= xl + fabs((xl - xr) / 2.0);
+        const bool low = pi.y <= yb + fabs((yt - yb) / 2.0);
+        int ox[4], oy[4];
+        ox[0] = 0; oy[0] = 0;
+        if (left && low) { ox[1] = -1; oy[1] = 0; ox[2] = 0; oy[2] = -1; ox[3] = -1; oy[3] = -1; }
+        else if (!left && low) { ox[1] = 1; oy[1] = 0; ox[2] = 0; oy[2] = -1; ox[3] = 1; oy[3] = -1; }
+        else if (!left) { ox[1] = 1; oy[1] = 1; ox[2] = 0; oy[2] = 1; ox[3] = 1; oy[3] = 0; }
+        else { ox[1] = 0; oy[1] = 1; ox[2] = -1; oy[2] = 1; ox[3] = -1; oy[3] = 0; }
+        for (int q = 0; q < 4; ++q) {
+            const int c = (cx + ox[q]) * g.ny + (cy + oy[q]);
+            sf = soil_cells(kc, cpor, pi, sstart[c], sstart[c + 1], spos, sm, sr, sf);
         }
     }
     if (sf > (1.0 - pp.por)) sf = 1.0 - pp.por;

@@ -45,8 +45,9 @@ __global__ void k_probe(int np, const double* __restrict__ px, const double* __r
     double term1 = 0.0, pr = 0.0;
     int count = 0;
     for (int q = 0; q < 9; ++q) {
-        const int c = (cx + ox[q]) * g.ny + (cy + oy[q]);
-        const int lo = bstart[2 * c], hi = bstart[2 * c + 2];
+        int blo, bhi;
+        column_bins(g, cx + ox[q], cy + oy[q], cy + oy[q], blo, bhi);
+        const int lo = bstart[blo], hi = bstart[bhi];
         for (int j = lo; j < hi; ++j) {
             if (s.tid[j] <= 2) continue;
             const double2 pj = s.pos[j];

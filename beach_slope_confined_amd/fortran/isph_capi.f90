@@ -22,7 +22,7 @@ module isph_capi
         real(c_double) :: csh, maxshift, lamfs, maxdivr, por, alpha, beta, d50, tol, inner_tol
         real(c_double) :: xlcutoff, ytcutoff, sig1, mu, rhomin, tschmidt, salt_xbuf
         integer(c_int32_t) :: kernel, drag_law, precision, maxit, inner_maxit, max_outer, max_nbr
-        integer(c_int32_t) :: shifting, adaptive_dt, sink, visc_law, salinity
+        integer(c_int32_t) :: shifting, adaptive_dt, sink, visc_law, salinity, cell_order, reserved
     end type isph_params
 
     interface

@@ -17,6 +17,8 @@ PREC_MIXED = 1
 PREC_MIXED_V64 = 2
 VISC_REFERENCE = 0
 VISC_CONSTANT = 1
+ORDER_CELL = 0
+ORDER_REFERENCE = 1
 
 FLAG_DEAD = 1
 FLAG_BUFFER = 2
@@ -45,6 +47,7 @@ _DOUBLES = [
 _INTS = [
     "kernel", "drag_law", "precision", "maxit", "inner_maxit", "max_outer",
     "max_nbr", "shifting", "adaptive_dt", "sink", "visc_law", "salinity",
+    "cell_order", "reserved",
 ]
 
 
@@ -80,6 +83,7 @@ def make_params(**kw):
         beta=1.1, d50=0.01, tol=1e-8, inner_tol=0.1, maxit=50000,
         inner_maxit=0, max_outer=200, max_nbr=64, shifting=1, adaptive_dt=0,
         sink=0, xlcutoff=-1e300, ytcutoff=1e300, sig1=0.2, mu=1e-3, salinity=0,
+        cell_order=ORDER_CELL, reserved=0,
         rhomin=1000.0, tschmidt=1.0, salt_xbuf=1.6,
     )
     defaults.update(kw)

@@ -62,6 +62,22 @@ extern "C" {
 #define ISPH_PREC_MIXED_V64 2  /* FP32 matrix values, FP64 Krylov vectors, FP64
                                   residual + reliable updates                        */
 
+/* Particle order kept by cellshift/cellshiftalt (isph_params.cell_order).
+ * ISPH_ORDER_CELL: by cell (column-major, y ascending), walls first, then
+ *   fluid, each in its previous order.
+ * ISPH_ORDER_REFERENCE: the reference's plist order after cellshift and
+ *   resetid (integrator.f90:250-391, isph.f90:88-113): cells numbered column
+ *   by column with y descending; in a cell the walls, then the fluid that
+ *   stayed, then arrivals from the cells below, above, right, left, upper
+ *   left, lower left, upper right and lower right, each in its previous
+ *   order; a fluid particle moves at most one cell per re-binning.  effpor
+ *   then visits the soil in porous.f90:127-176's 2x2-cell order.  Needs
+ *   cell_size >= 4h, and cell boundaries at the reference's: xmin and ymin
+ *   multiples of cell_size (its cells start at x = 0 and y = 0,
+ *   setup.f90:120-123). */
+#define ISPH_ORDER_CELL      0
+#define ISPH_ORDER_REFERENCE 1
+
 /* particle flag bits (particle.f90:123-124) */
 #define ISPH_FLAG_DEAD      1
 #define ISPH_FLAG_BUFFER    2
@@ -119,6 +135,8 @@ typedef struct isph_params {
     int32_t salinity;    /* 1: isph_step() runs the salt-water step of main.f90:199-263
                             (transport substeps first, densityupdate after the
                             shift, eddyvis after timestep); 0: main.f90:41-66    */
+    int32_t cell_order;  /* ISPH_ORDER_*: particle order kept by the re-binnings */
+    int32_t reserved;    /* keeps the struct a multiple of 8 bytes; set to 0    */
 } isph_params;
 
 typedef struct isph_ctx isph_ctx;

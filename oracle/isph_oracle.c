@@ -59,7 +59,8 @@ struct ora_ctx {
     int32_t *tid, *flags, *pid;
     /* cell grid */
     int32_t nx, ny;
-    int64_t nbins;          /* 2 * nx * ny */
+    int32_t per;            /* bins per cell: 2 (cell order) or 10 (reference order) */
+    int64_t nbins;          /* per * nx * ny */
     int32_t *key;           /* sorted key per particle */
     int32_t *bstart;        /* nbins + 2 */
     int32_t *perm;
@@ -286,6 +287,10 @@ static int check_params(ora_ctx* c, const isph_params* prm) {
     if (!(prm->h > 0.0) || !(prm->cell_size >= 2.0 * prm->h) ||
         !(prm->xmax > prm->xmin) || !(prm->ymax > prm->ymin) || prm->max_nbr <= 0)
         return set_err(c, ISPH_E_ARG, "invalid geometry parameters");
+    if (prm->cell_order != ISPH_ORDER_CELL && prm->cell_order != ISPH_ORDER_REFERENCE)
+        return set_err(c, ISPH_E_ARG, "unknown cell_order");
+    if (prm->cell_order == ISPH_ORDER_REFERENCE && !(prm->cell_size >= 4.0 * prm->h))
+        return set_err(c, ISPH_E_ARG, "the reference order needs cell_size >= 4h");
     int32_t nx, ny;
     grid_dims(prm, &nx, &ny);
     if ((double)nx * (double)ny * 2.0 > 2.0e9) return set_err(c, ISPH_E_ARG, "cell grid too large");
@@ -324,9 +329,10 @@ int ora_create(const isph_params* prm, int32_t cap, int32_t cap_soil, ora_ctx** 
     c->col = (int32_t*)xmalloc(ni * (size_t)(prm->max_nbr + 1));
     c->val = (double*)xmalloc(nd * (size_t)(prm->max_nbr + 1));
     grid_dims(prm, &c->nx, &c->ny);
-    c->nbins = 2 * (int64_t)c->nx * c->ny;
+    c->per = prm->cell_order ? 10 : 2;
+    c->nbins = c->per * (int64_t)c->nx * c->ny;
     c->bstart = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(c->nbins + 2));
-    c->sstart = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(c->nbins / 2 + 1));
+    c->sstart = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)((int64_t)c->nx * c->ny + 1));
     size_t ns = sizeof(double) * (size_t)(cap_soil ? cap_soil : 1);
     c->sx = (double*)xmalloc(ns);
     c->sy = (double*)xmalloc(ns);
@@ -359,8 +365,9 @@ int ora_set_params(ora_ctx* c, const isph_params* prm) {
     int rc = check_params(c, prm);
     if (rc) return rc;
     grid_dims(prm, &nx, &ny);
-    if (nx != c->nx || ny != c->ny || prm->max_nbr != c->prm.max_nbr)
-        return set_err(c, ISPH_E_ARG, "grid and max_nbr are fixed at create time");
+    if (nx != c->nx || ny != c->ny || prm->max_nbr != c->prm.max_nbr ||
+        prm->cell_order != c->prm.cell_order)
+        return set_err(c, ISPH_E_ARG, "grid, cell order and max_nbr are fixed at create time");
     c->prm = *prm;
     return ISPH_OK;
 }
@@ -381,6 +388,46 @@ static inline int32_t cell_of(const ora_ctx* c, double x, double y, int* oob) {
     return cx * c->ny + cy;
 }
 
+/* Bin layout.  Cell order: bin = 2 (cx ny + cy) + fluid bit.  Reference
+ * order: the reference's resetid numbering (isph.f90:96-109: cells column by
+ * column, row i = 1 at the top, so y descending) times 10 plus a class:
+ * 0 walls, 1 fluid that stayed, 2..9 the arrivals in the gather order of
+ * integrator.f90:318-389 (from below, above, right, left, upper left, lower
+ * left, upper right, lower right). */
+static inline int32_t cell_rank(const ora_ctx* c, int32_t cx, int32_t cy) {
+    return c->prm.cell_order ? cx * c->ny + (c->ny - 1 - cy) : cx * c->ny + cy;
+}
+
+static inline void key_cell(const ora_ctx* c, int32_t key, int32_t* cx, int32_t* cy) {
+    if (c->prm.cell_order) {
+        int32_t r = key / 10;
+        *cx = r / c->ny;
+        *cy = c->ny - 1 - (r - *cx * c->ny);
+    } else {
+        *cx = (key >> 1) / c->ny;
+        *cy = (key >> 1) - *cx * c->ny;
+    }
+}
+
+/* first and one-past-last bin of the cells ylo..yhi of column cx */
+static inline void column_bins(const ora_ctx* c, int32_t cx, int32_t ylo, int32_t yhi,
+                               int32_t* lo, int32_t* hi) {
+    if (c->prm.cell_order) {
+        *lo = 10 * (cx * c->ny + (c->ny - 1 - yhi));
+        *hi = 10 * (cx * c->ny + (c->ny - 1 - ylo)) + 10;
+    } else {
+        *lo = 2 * (cx * c->ny + ylo);
+        *hi = 2 * (cx * c->ny + yhi) + 2;
+    }
+}
+
+/* cellshift's transfer list of a fluid particle that crossed (dx, dy)
+ * cells (integrator.f90:271-310), as the class of its arrival */
+static inline int32_t move_class(int32_t dx, int32_t dy) {
+    static const int32_t t[3][3] = {{8, 3, 6}, {4, 1, 5}, {9, 2, 7}};
+    return t[dy + 1][dx + 1];
+}
+
 /* permute every per-particle array: new[i] = old[perm[i]] */
 static void apply_perm(ora_ctx* c, int32_t nnew) {
     double* dp[] = {c->x, c->y, c->vx, c->vy, c->xs, c->ys, c->vxs, c->vys, c->mass,
@@ -399,11 +446,13 @@ static void apply_perm(ora_ctx* c, int32_t nnew) {
     }
 }
 
-/* Stable counting sort by key = 2*cell + fluid bit; deleted particles get
- * key nbins and are dropped.  Plays the role of cellshift/cellshiftalt's
- * re-binning (integrator.f90:32-391) plus resetid (isph.f90:88-113): walls
- * first in each cell, then fluid, each in previous order. */
-static int sort_particles(ora_ctx* c, int with_sink) {
+/* Stable counting sort by bin key; deleted particles get key nbins and are
+ * dropped.  Plays the role of cellshift/cellshiftalt's re-binning
+ * (integrator.f90:32-391) plus resetid (isph.f90:88-113).  In the reference
+ * order a fluid particle goes to the cell next to its previous one in the
+ * direction it crossed (one cell at most, :271-310); `first`: no previous
+ * binning. */
+static int sort_particles(ora_ctx* c, int with_sink, int first) {
     const isph_params* prm = &c->prm;
     int oob = 0;
     int64_t nb = c->nbins;
@@ -416,7 +465,23 @@ static int sort_particles(ora_ctx* c, int with_sink) {
             c->x[i] <= prm->xlcutoff && c->y[i] > prm->ytcutoff) {
             k = (int32_t)nb;
         } else {
-            k = 2 * cell_of(c, c->x[i], c->y[i], &oob) + fl;
+            int32_t cell = cell_of(c, c->x[i], c->y[i], &oob);
+            if (!c->prm.cell_order) {
+                k = 2 * cell + fl;
+            } else {
+                int32_t cx = cell / c->ny, cy = cell % c->ny, cls = fl;
+                if (fl && !first) {
+                    int32_t ox, oy;
+                    key_cell(c, c->key[i], &ox, &oy);
+                    int32_t dx = cx - ox, dy = cy - oy;
+                    dx = dx < -1 ? -1 : (dx > 1 ? 1 : dx);
+                    dy = dy < -1 ? -1 : (dy > 1 ? 1 : dy);
+                    cx = ox + dx;
+                    cy = oy + dy;
+                    cls = move_class(dx, dy);
+                }
+                k = 10 * cell_rank(c, cx, cy) + cls;
+            }
         }
         c->tmpi[i] = k;
         cnt[k + 1] += 1;
@@ -467,7 +532,7 @@ int ora_set_particles(ora_ctx* c, int32_t n, const double* x, const double* y,
     }
     c->umax = 0.0; c->numax = 0.0; c->t = 0.0; c->dt = c->prm.dt;
     c->dtsol = c->prm.dt; c->solsteps = 0;  /* solsteps = 0, initialize.f90:56 */
-    return sort_particles(c, 0);
+    return sort_particles(c, 0, 1);
 }
 
 int ora_get_particles(ora_ctx* c, int32_t* n, double* x, double* y, double* vx,
@@ -493,7 +558,7 @@ int ora_get_particles(ora_ctx* c, int32_t* n, double* x, double* y, double* vx,
 int ora_set_soil(ora_ctx* c, int32_t ns, const double* x, const double* y,
                  const double* mass, const double* density) {
     if (ns < 0 || ns > c->cap_soil) return set_err(c, ISPH_E_ARG, "ns exceeds soil capacity");
-    int64_t nc = c->nbins / 2;
+    int64_t nc = (int64_t)c->nx * c->ny;
     int32_t* cnt = c->sstart;
     memset(cnt, 0, sizeof(int32_t) * (size_t)(nc + 1));
     int32_t* k = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ns ? ns : 1));
@@ -617,8 +682,8 @@ int ora_projection(ora_ctx* c) {
     return ISPH_OK;
 }
 
-int ora_cellshift(ora_ctx* c) { return sort_particles(c, 0); }
-int ora_cellshiftalt(ora_ctx* c) { return sort_particles(c, 1); }
+int ora_cellshift(ora_ctx* c) { return sort_particles(c, 0, 0); }
+int ora_cellshiftalt(ora_ctx* c) { return sort_particles(c, 1, 0); }
 /* resetid (isph.f90:88-113): the sorted position is the matid already. */
 int ora_resetid(ora_ctx* c) { (void)c; return ISPH_OK; }
 
@@ -630,18 +695,18 @@ int ora_neighbour(ora_ctx* c) {
     const double h2x = 2.0 * prm->h;
     const double thr = prm->lamfs * prm->maxdivr;
     const int32_t M = prm->max_nbr;
-    const int32_t ny = c->ny;
     int over = 0;
 #pragma omp parallel for schedule(static) reduction(| : over)
     for (int32_t i = 0; i < c->n; ++i) {
-        int32_t cell = c->key[i] >> 1;
-        int32_t cx = cell / ny, cy = cell % ny;
+        int32_t cx, cy;
+        key_cell(c, c->key[i], &cx, &cy);
         int32_t cnt = 0;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
         const double xi = c->x[i], yi = c->y[i];
         for (int32_t dc = -1; dc <= 1; ++dc) {
-            int32_t lo = c->bstart[2 * ((cx + dc) * ny + cy - 1)];
-            int32_t hi = c->bstart[2 * ((cx + dc) * ny + cy + 1) + 2];
+            int32_t blo, bhi;
+            column_bins(c, cx + dc, cy - 1, cy + 1, &blo, &bhi);
+            int32_t lo = c->bstart[blo], hi = c->bstart[bhi];
             for (int32_t j = lo; j < hi; ++j) {
                 if (j == i) continue;
                 double dx = xi - c->x[j], dy = yi - c->y[j];
@@ -706,10 +771,37 @@ int ora_effpor(ora_ctx* c) {
         oob |= o;
         int32_t cx = cell / ny, cy = cell % ny;
         double sf = 0.0;
-        for (int32_t dc = -1; dc <= 1; ++dc) {
-            int32_t lo = c->sstart[(cx + dc) * ny + cy - 1];
-            int32_t hi = c->sstart[(cx + dc) * ny + cy + 2];
-            for (int32_t s = lo; s < hi; ++s) {
+        /* soil cells visited: the 3x3 block column by column (cell order), or
+         * porous.f90:127-176 in the reference order: the particle's own cell
+         * (its plist cell), then cell2..cell4 of the 2x2 block on its side of
+         * the cell midlines (x <= mid: left; y <= mid: lower) */
+        int32_t cells[9], nc = 0;
+        if (!prm->cell_order) {
+            for (int32_t dc = -1; dc <= 1; ++dc)
+                for (int32_t dr = -1; dr <= 1; ++dr) cells[nc++] = (cx + dc) * ny + cy + dr;
+        } else {
+            key_cell(c, c->key[i], &cx, &cy);
+            double xleft = prm->xmin + (double)(cx - 1) * prm->cell_size;
+            double xright = prm->xmin + (double)cx * prm->cell_size;
+            double ybot = prm->ymin + (double)(cy - 1) * prm->cell_size;
+            double ytop = prm->ymin + (double)cy * prm->cell_size;
+            double xmid = xleft + fabs((xleft - xright) / 2.0);
+            double ymid = ybot + fabs((ytop - ybot) / 2.0);
+            int32_t d[3][2];
+            if (xi <= xmid && yi <= ymid) {          /* (i,j-1), (i+1,j), (i+1,j-1) */
+                d[0][0] = -1; d[0][1] = 0; d[1][0] = 0; d[1][1] = -1; d[2][0] = -1; d[2][1] = -1;
+            } else if (xi > xmid && yi <= ymid) {    /* (i,j+1), (i+1,j), (i+1,j+1) */
+                d[0][0] = 1; d[0][1] = 0; d[1][0] = 0; d[1][1] = -1; d[2][0] = 1; d[2][1] = -1;
+            } else if (xi > xmid && yi > ymid) {     /* (i-1,j+1), (i-1,j), (i,j+1) */
+                d[0][0] = 1; d[0][1] = 1; d[1][0] = 0; d[1][1] = 1; d[2][0] = 1; d[2][1] = 0;
+            } else {                                 /* (i-1,j), (i-1,j-1), (i,j-1) */
+                d[0][0] = 0; d[0][1] = 1; d[1][0] = -1; d[1][1] = 1; d[2][0] = -1; d[2][1] = 0;
+            }
+            cells[nc++] = cx * ny + cy;
+            for (int q = 0; q < 3; ++q) cells[nc++] = (cx + d[q][0]) * ny + cy + d[q][1];
+        }
+        for (int32_t q = 0; q < nc; ++q) {
+            for (int32_t s = c->sstart[cells[q]]; s < c->sstart[cells[q] + 1]; ++s) {
                 double dx = xi - c->sx[s], dy = yi - c->sy[s];
                 double r = sqrt(dx * dx + dy * dy);
                 if (r < h2x)
@@ -1651,8 +1743,9 @@ int ora_probe(ora_ctx* c, int32_t np, const double* px, const double* py, double
         double term1 = 0.0, pr = 0.0;
         int32_t count = 0;
         for (int q = 0; q < 9; ++q) {
-            int32_t cc = (cx + ox[q]) * c->ny + (cy + oy[q]);
-            for (int32_t j = c->bstart[2 * cc]; j < c->bstart[2 * cc + 2]; ++j) {
+            int32_t blo, bhi;
+            column_bins(c, cx + ox[q], cy + oy[q], cy + oy[q], &blo, &bhi);
+            for (int32_t j = c->bstart[blo]; j < c->bstart[bhi]; ++j) {
                 if (c->tid[j] <= 2) continue;
                 double dx = px[k] - c->x[j], dy = py[k] - c->y[j];
                 double r = sqrt(dx * dx + dy * dy);
