@@ -62,6 +62,42 @@ def make_objective(probes=(), gauges=()):
     return o
 
 
+MAX_ZONES = 4
+
+
+class IsphBufferZone(ctypes.Structure):
+    """isph_buffer_zone: a buffer zone's template, entry cells and schedule."""
+    _fields_ = [("ntemplate", ctypes.c_int32), ("ncells", ctypes.c_int32),
+                ("x", _c_dp), ("y", _c_dp), ("vx", _c_dp), ("vy", _c_dp), ("pressure", _c_dp),
+                ("flags", _c_ip), ("cells", _c_ip),
+                ("mass", ctypes.c_double), ("density", ctypes.c_double),
+                ("ovol", ctypes.c_double), ("con", ctypes.c_double),
+                ("period", ctypes.c_double), ("xcut", ctypes.c_double),
+                ("remove_right", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+def make_zones(zones):
+    """ctypes zones from dicts (x, y, vx, vy, pressure, flags, cells (k, 2),
+    mass, density, ovol, con, period, xcut, remove_right); returns the array
+    and the numpy buffers it points into (keep both alive for the call)."""
+    arr = (IsphBufferZone * max(len(zones), 1))()
+    keep = []
+    for k, z in enumerate(zones):
+        a = {f: np.ascontiguousarray(z[f], dtype=np.float64) for f in ("x", "y", "vx", "vy",
+                                                                      "pressure")}
+        fl = np.ascontiguousarray(z["flags"], dtype=np.int32)
+        cells = np.ascontiguousarray(np.asarray(z["cells"], dtype=np.int32).reshape(-1, 2))
+        keep += list(a.values()) + [fl, cells]
+        q = arr[k]
+        q.ntemplate, q.ncells = len(a["x"]), len(cells)
+        q.x, q.y, q.vx, q.vy, q.pressure = (_dp(a[f]) for f in ("x", "y", "vx", "vy", "pressure"))
+        q.flags, q.cells = _ip(fl), _ip(cells)
+        for f in ("mass", "density", "ovol", "con", "period", "xcut"):
+            setattr(q, f, float(z[f]))
+        q.remove_right, q.reserved = int(z["remove_right"]), 0
+    return arr, keep
+
+
 class IsphError(RuntimeError):
     def __init__(self, code, where, msg):
         self.code = code
@@ -100,6 +136,9 @@ def declare(lib, prefix):
         "salt_init": [P, ctypes.c_double, ctypes.c_double, ctypes.c_double],
         "get_substeps": [P, _c_ip, _c_dp],
         "probe": [P, i32, _c_dp, _c_dp, _c_dp],
+        "set_buffer_zones": [P, i32, ctypes.POINTER(IsphBufferZone), ctypes.c_double],
+        "remove_buffer": [P, i32],
+        "insert_buffer": [P, i32],
     }
     for name in SUBROUTINES:
         sig[name] = [P]
@@ -352,6 +391,19 @@ class Sim:
         out = np.zeros(len(px))
         self._check(self._fn("probe")(self._h, len(px), _dp(px), _dp(py), _dp(out)), "probe")
         return out
+
+    # -- buffer zones (internal.f90:475-614) -----------------------------------
+    def set_buffer_zones(self, zones, time_shift=0.0):
+        arr, keep = make_zones(zones)
+        self._check(self._fn("set_buffer_zones")(self._h, len(zones), arr, float(time_shift)),
+                    "set_buffer_zones")
+        del keep
+
+    def remove_buffer(self, zone):
+        self._check(self._fn("remove_buffer")(self._h, int(zone)), "remove_buffer")
+
+    def insert_buffer(self, zone):
+        self._check(self._fn("insert_buffer")(self._h, int(zone)), "insert_buffer")
 
     def substeps(self):
         """(solsteps, dtsol) of the next step's transport."""

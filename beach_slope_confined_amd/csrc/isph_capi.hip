@@ -612,8 +612,17 @@ int run_solve(isph_ctx* c) {
 int run_step(isph_ctx* c) {
     int rc;
     const bool salt = c->prm.salinity != 0;
+    double told = 0.0, tnew = 0.0;
+    if (c->nzones) {  // the buffer schedule compares the times around the advance
+        HIPCHK(c, hipMemcpyAsync(c->ss_host, c->ss, sizeof(SimScalars), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        told = c->ss_host->t;
+        tnew = told + c->ss_host->dt;  // k_advance_time's sum
+    }
     k_advance_time<<<1, 64, 0, c->stream>>>(c->ss);
     if (salt && (rc = run_salt_substeps(c))) return rc;
+    if (c->nzones && (rc = run_buffer_schedule(c, told, tnew))) return rc;
     if ((rc = run_projection(c))) return rc;
     if ((rc = sort_particles(c, false, false))) return rc;
     if ((rc = run_neighbour(c))) return rc;
@@ -691,7 +700,9 @@ int isph_create(const isph_params* prm, int32_t cap, int32_t cap_soil, isph_ctx*
     rc |= dalloc(c, &c->rank, N);
     rc |= dalloc(c, &c->perm, N);
     rc |= dalloc(c, &c->bincnt, (size_t)nb + 2);
-    rc |= dalloc(c, &c->scanbuf, (size_t)nblk(nb + 2, SCAN_BLOCK) + 1);
+    // block sums of the scans: over the bins of a re-binning and over the
+    // particles of a buffer removal
+    rc |= dalloc(c, &c->scanbuf, (size_t)nblk(std::max<int64_t>(nb + 2, (int64_t)N), SCAN_BLOCK) + 1);
     rc |= dalloc(c, &c->nbr, N * M);
     rc |= dalloc(c, &c->cnt, N);
     rc |= dalloc(c, &c->val64, N * M);
@@ -756,6 +767,7 @@ int isph_destroy(isph_ctx* c) {
     if (c->sc_host) (void)hipHostFree(c->sc_host);
     if (c->ss_host) (void)hipHostFree(c->ss_host);
     adj_free(c);
+    free_zones(c);
     for (auto& sl : c->slots) {
         if (!sl.v.pos) continue;
         free_state(sl.v);
@@ -838,6 +850,8 @@ int isph_set_particles(isph_ctx* c, int32_t n, const double* x, const double* y,
     SimScalars ss0 = {0.0, 0.0, c->prm.dt, 0.0, c->prm.dt, 0, 0};
     HIPCHK(c, hipMemcpyAsync(c->ss, &ss0, sizeof(ss0), H2D, c->stream));
     c->n = n;
+    c->next_pid = 0;
+    for (int i = 0; i < n; ++i) c->next_pid = std::max(c->next_pid, pid[i] + 1);
     int rc = sort_particles(c, false, true);
     if (rc) return rc;
     return sync_check(c);

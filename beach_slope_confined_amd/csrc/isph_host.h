@@ -36,6 +36,18 @@ struct Profiler {
 
 struct AdjointState;  // isph_adjoint.hip
 
+// a buffer zone as installed by isph_set_buffer_zones (isph_buffers.hip)
+struct BufferZone {
+    int nins = 0;                  // template particles that lie in the entry cells
+    double2 *pos = nullptr, *vel = nullptr;
+    double* p = nullptr;
+    int *flags = nullptr, *key = nullptr;  // template flags and bin keys
+    unsigned char* cellmask = nullptr;     // per cell: entry cell of the zone
+    double mass = 0, density = 0, ovol = 0, con = 0, period = 0, xcut = 0;
+    int remove_right = 0;
+    long long counter = 1;         // the reference's iter1 / iter2
+};
+
 }  // namespace host
 }  // namespace isph
 
@@ -87,6 +99,10 @@ struct isph_ctx {
         bool used = false;
     } slots[64];
     isph::host::AdjointState* adj = nullptr;
+    isph::host::BufferZone zones[ISPH_MAX_ZONES];
+    int nzones = 0;
+    double zone_shift = 0.0;
+    int next_pid = 0;              // pid of the next inserted particle
     std::string msg;
 };
 
@@ -154,6 +170,11 @@ int run_scalart(isph_ctx* c);
 int run_scalarupdate(isph_ctx* c, double stime, bool from_ss);
 int run_densityupdate(isph_ctx* c);
 int run_salt_substeps(isph_ctx* c);
+
+// buffer zones (isph_buffers.hip)
+int run_buffer_schedule(isph_ctx* c, double told, double tnew);
+void free_zones(isph_ctx* c);
+int scan(isph_ctx* c, int* data, int64_t count);
 
 }  // namespace host
 }  // namespace isph

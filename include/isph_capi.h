@@ -262,6 +262,39 @@ int isph_salt_init(isph_ctx* ctx, double xl, double yl, double line_grad);
  * first timestep, as the reference's solsteps = 0 (initialize.f90:56). */
 int isph_get_substeps(isph_ctx* ctx, int32_t* solsteps, double* dtsol);
 
+/* ---- inflow/outflow buffers (internal.f90:475-614, main.f90:103-121,218-236) */
+/* A buffer zone: its entry cells, the template lattice inserted into them
+ * every `period` seconds of simulated time, and the line past which its
+ * buffer particles leave.  insert_buffer1/2 append the template particles
+ * that lie in the entry cells as fluid (tid 3) with the zone's mass,
+ * density = oden, ovol and con; remove_buffer1 drops the buffer particles of
+ * the entry cells with x >= xcut (remove_right = 1), remove_buffer2 those
+ * with x <= xcut (remove_right = 0). */
+#define ISPH_MAX_ZONES 4
+typedef struct isph_buffer_zone {
+    int32_t ntemplate;     /* template particles                               */
+    int32_t ncells;        /* entry cells                                      */
+    const double *x, *y, *vx, *vy, *pressure;   /* template state (ntemplate)  */
+    const int32_t* flags;  /* template flags: ISPH_FLAG_BUFFER, ISPH_FLAG_DEAD */
+    const int32_t* cells;  /* ncells (cx, cy) pairs on the context's cell grid:
+                              cx = floor((x - xmin)/cell_size) + 1, likewise cy */
+    double mass, density, ovol, con;  /* of every inserted particle            */
+    double period;         /* insertion period (the reference's ins_1, ins_2)  */
+    double xcut;           /* removal line (rv_buf_r, rv_buf_l)                */
+    int32_t remove_right;  /* 1: x >= xcut leaves; 0: x <= xcut leaves         */
+    int32_t reserved;      /* set to 0                                         */
+} isph_buffer_zone;
+/* Installs up to ISPH_MAX_ZONES zones (0 removes them) and restarts their
+ * schedules: at each isph_step, with told and t the times before and after
+ * its advance, zone k runs remove then insert when
+ * told + time_shift < m_k period_k <= t + time_shift, and m_k then counts on
+ * (m_k = 1 after this call, main.f90:93-94,194-196).  In a salt-water step
+ * this happens after the transport substeps, as main.f90:210-236 orders it. */
+int isph_set_buffer_zones(isph_ctx* ctx, int32_t nzones, const isph_buffer_zone* zones,
+                          double time_shift);
+int isph_remove_buffer(isph_ctx* ctx, int32_t zone);   /* remove_buffer1/2 */
+int isph_insert_buffer(isph_ctx* ctx, int32_t zone);   /* insert_buffer1/2 */
+
 /* ---- text output and probes (output.f90, probe.f90) --------------------- */
 /* Fortran edit descriptors Fw.d and ESw.d as a Fortran run-time library
  * prints them (correct rounding, ties to even; w asterisks when the value

@@ -80,6 +80,18 @@ struct ora_ctx {
     float *f[8];
     double umax, numax, dt, t, relres, dtsol;
     int32_t iters, outer, solsteps;
+    /* buffer zones (internal.f90:475-614): the template particles inside the
+     * entry cells in insertion order, a per-cell entry mask, the schedule */
+    struct {
+        int32_t nins, remove_right;
+        double *x, *y, *vx, *vy, *p;
+        int32_t *flags, *key;
+        unsigned char* mask;
+        double mass, density, ovol, con, period, xcut;
+        int64_t counter;
+    } zone[ISPH_MAX_ZONES];
+    int32_t nzones, next_pid;
+    double zone_shift;
     char err[256];
 };
 
@@ -344,8 +356,18 @@ int ora_create(const isph_params* prm, int32_t cap, int32_t cap_soil, ora_ctx** 
     return ISPH_OK;
 }
 
+static void free_zones(ora_ctx* c) {
+    for (int k = 0; k < ISPH_MAX_ZONES; ++k) {
+        free(c->zone[k].x); free(c->zone[k].y); free(c->zone[k].vx); free(c->zone[k].vy);
+        free(c->zone[k].p); free(c->zone[k].flags); free(c->zone[k].key); free(c->zone[k].mask);
+        memset(&c->zone[k], 0, sizeof(c->zone[k]));
+    }
+    c->nzones = 0;
+}
+
 int ora_destroy(ora_ctx* c) {
     if (!c) return ISPH_OK;
+    free_zones(c);
     double* dp[] = {c->x, c->y, c->vx, c->vy, c->xs, c->ys, c->vxs, c->vys, c->mass,
                     c->rho, c->oden, c->p, c->por, c->R, c->nut, c->tke, c->varts,
                     c->vel, c->divr, c->tmpd, c->rhs, c->L, c->val, c->sx, c->sy,
@@ -532,6 +554,9 @@ int ora_set_particles(ora_ctx* c, int32_t n, const double* x, const double* y,
     }
     c->umax = 0.0; c->numax = 0.0; c->t = 0.0; c->dt = c->prm.dt;
     c->dtsol = c->prm.dt; c->solsteps = 0;  /* solsteps = 0, initialize.f90:56 */
+    c->next_pid = 0;
+    for (int32_t i = 0; i < n; ++i)
+        if (pid[i] + 1 > c->next_pid) c->next_pid = pid[i] + 1;
     return sort_particles(c, 0, 1);
 }
 
@@ -1768,14 +1793,147 @@ int ora_get_substeps(ora_ctx* c, int32_t* solsteps, double* dtsol) {
     return ISPH_OK;
 }
 
+/* ---------------------------------------------------------------------- */
+/* internal.f90:475-614 buffer zones, main.f90:103-121,218-236 schedule    */
+/* ---------------------------------------------------------------------- */
+int ora_set_buffer_zones(ora_ctx* c, int32_t nz, const isph_buffer_zone* zs, double shift) {
+    if (nz < 0 || nz > ISPH_MAX_ZONES || (nz && !zs)) return set_err(c, ISPH_E_ARG, "bad zones");
+    free_zones(c);
+    const int64_t ncell = (int64_t)c->nx * c->ny;
+    for (int k = 0; k < nz; ++k) {
+        const isph_buffer_zone* q = &zs[k];
+        if (!(q->period > 0.0)) { free_zones(c); return set_err(c, ISPH_E_ARG, "invalid buffer zone"); }
+        unsigned char* mask = (unsigned char*)xmalloc((size_t)ncell);
+        int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)ncell);
+        for (int64_t e = 0; e < ncell; ++e) order[e] = -1;
+        for (int32_t e = 0; e < q->ncells; ++e) {
+            int32_t cx = q->cells[2 * e], cy = q->cells[2 * e + 1];
+            if (cx < 1 || cx > c->nx - 2 || cy < 1 || cy > c->ny - 2) {
+                free(mask); free(order); free_zones(c);
+                return set_err(c, ISPH_E_ARG, "buffer entry cell outside the cell grid");
+            }
+            mask[cx * c->ny + cy] = 1;
+            if (order[cx * c->ny + cy] < 0) order[cx * c->ny + cy] = e;
+        }
+        /* template particles in the entry cells, entry cell by entry cell,
+         * each in template order (internal.f90:482-504) */
+        int32_t* sel = (int32_t*)malloc(sizeof(int32_t) * (size_t)(q->ntemplate + 1));
+        int32_t m = 0;
+        for (int32_t e = 0; e < q->ncells; ++e) {
+            int32_t cell = q->cells[2 * e] * c->ny + q->cells[2 * e + 1];
+            if (order[cell] != e) continue;
+            for (int32_t t = 0; t < q->ntemplate; ++t) {
+                double fx = floor((q->x[t] - c->prm.xmin) / c->prm.cell_size);
+                double fy = floor((q->y[t] - c->prm.ymin) / c->prm.cell_size);
+                if (!(fx >= 0.0) || !(fy >= 0.0) || fx > c->nx - 3 || fy > c->ny - 3) continue;
+                if (((int32_t)fx + 1) * c->ny + ((int32_t)fy + 1) == cell) sel[m++] = t;
+            }
+        }
+        c->zone[k].x = (double*)xmalloc(sizeof(double) * (size_t)m);
+        c->zone[k].y = (double*)xmalloc(sizeof(double) * (size_t)m);
+        c->zone[k].vx = (double*)xmalloc(sizeof(double) * (size_t)m);
+        c->zone[k].vy = (double*)xmalloc(sizeof(double) * (size_t)m);
+        c->zone[k].p = (double*)xmalloc(sizeof(double) * (size_t)m);
+        c->zone[k].flags = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)m);
+        c->zone[k].key = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)m);
+        for (int32_t u = 0; u < m; ++u) {
+            int32_t t = sel[u];
+            c->zone[k].x[u] = q->x[t]; c->zone[k].y[u] = q->y[t];
+            c->zone[k].vx[u] = q->vx[t]; c->zone[k].vy[u] = q->vy[t];
+            c->zone[k].p[u] = q->pressure[t]; c->zone[k].flags[u] = q->flags[t];
+            int32_t cx = (int32_t)floor((q->x[t] - c->prm.xmin) / c->prm.cell_size) + 1;
+            int32_t cy = (int32_t)floor((q->y[t] - c->prm.ymin) / c->prm.cell_size) + 1;
+            c->zone[k].key[u] = c->prm.cell_order ? 10 * cell_rank(c, cx, cy) + 1
+                                                  : 2 * (cx * c->ny + cy) + 1;
+        }
+        free(sel);
+        free(order);
+        c->zone[k].mask = mask;
+        c->zone[k].nins = m;
+        c->zone[k].mass = q->mass; c->zone[k].density = q->density;
+        c->zone[k].ovol = q->ovol; c->zone[k].con = q->con;
+        c->zone[k].period = q->period; c->zone[k].xcut = q->xcut;
+        c->zone[k].remove_right = q->remove_right;
+        c->zone[k].counter = 1;
+        c->nzones = k + 1;
+    }
+    c->zone_shift = shift;
+    return ISPH_OK;
+}
+
+/* remove_buffer1/2: buffer particles of the entry cells past the line leave,
+ * the rest keep their order */
+int ora_remove_buffer(ora_ctx* c, int32_t zi) {
+    if (zi < 0 || zi >= c->nzones) return set_err(c, ISPH_E_ARG, "no such zone");
+    int32_t m = 0;
+    for (int32_t i = 0; i < c->n; ++i) {
+        int32_t cx, cy;
+        key_cell(c, c->key[i], &cx, &cy);
+        int keep = 1;
+        if (c->zone[zi].mask[cx * c->ny + cy] && (c->flags[i] & ISPH_FLAG_BUFFER))
+            keep = c->zone[zi].remove_right ? (c->x[i] < c->zone[zi].xcut)
+                                            : (c->x[i] > c->zone[zi].xcut);
+        if (keep) c->perm[m++] = i;
+    }
+    if (m == c->n) return ISPH_OK;
+    for (int32_t u = 0; u < m; ++u) c->tmpi[u] = c->key[c->perm[u]];
+    memcpy(c->key, c->tmpi, sizeof(int32_t) * (size_t)m);
+    apply_perm(c, m);
+    c->n = m;
+    c->have_nbr = 0;
+    c->assembled = 0;
+    return ISPH_OK;
+}
+
+/* insert_buffer1/2: the zone's particles appended as fluid */
+int ora_insert_buffer(ora_ctx* c, int32_t zi) {
+    if (zi < 0 || zi >= c->nzones) return set_err(c, ISPH_E_ARG, "no such zone");
+    const int32_t m = c->zone[zi].nins, n0 = c->n;
+    if ((int64_t)n0 + m > c->cap) return set_err(c, ISPH_E_CAPACITY, "buffer insertion exceeds the particle capacity");
+    for (int32_t u = 0; u < m; ++u) {
+        int32_t i = n0 + u;
+        double vx = c->zone[zi].vx[u], vy = c->zone[zi].vy[u];
+        c->x[i] = c->zone[zi].x[u]; c->y[i] = c->zone[zi].y[u];
+        c->vx[i] = vx; c->vy[i] = vy;
+        c->xs[i] = 0.0; c->ys[i] = 0.0; c->vxs[i] = 0.0; c->vys[i] = 0.0;
+        c->mass[i] = c->zone[zi].mass; c->rho[i] = c->zone[zi].density;
+        c->oden[i] = c->zone[zi].density; c->p[i] = c->zone[zi].p[u];
+        c->por[i] = 1.0; c->R[i] = 0.0; c->nut[i] = 0.0; c->tke[i] = 0.0; c->varts[i] = 0.0;
+        c->vel[i] = sqrt(vx * vx + vy * vy);
+        c->con[i] = c->zone[zi].con; c->ovol[i] = c->zone[zi].ovol;
+        c->tid[i] = 3; c->flags[i] = c->zone[zi].flags[u]; c->pid[i] = c->next_pid + u;
+        c->key[i] = c->zone[zi].key[u];
+    }
+    c->n = n0 + m;
+    c->next_pid += m;
+    c->have_nbr = 0;
+    c->assembled = 0;
+    return ISPH_OK;
+}
+
+static int buffer_schedule(ora_ctx* c, double told, double tnew) {
+    for (int k = 0; k < c->nzones; ++k) {
+        double at = (double)c->zone[k].counter * c->zone[k].period;
+        if ((told + c->zone_shift) < at && (tnew + c->zone_shift) >= at) {
+            int rc = ora_remove_buffer(c, k);
+            if (rc) return rc;
+            if ((rc = ora_insert_buffer(c, k))) return rc;
+            c->zone[k].counter += 1;
+        }
+    }
+    return ISPH_OK;
+}
+
 /* One step in the order of main.f90:41-66 (t advances first, :42-43); with
- * salinity in the order of main.f90:199-263 without the buffers of :217-235:
+ * salinity in the order of main.f90:199-263 (buffer zones after the
+ * transport, :217-235):
  * the transport substeps on the previous step's lists first (:204-215),
  * densityupdate after the shift (:258), eddyvis after timestep (:260). */
 int ora_step(ora_ctx* c, int32_t nsteps) {
     int rc;
     const int salt = c->prm.salinity != 0;
     for (int32_t s = 0; s < nsteps; ++s) {
+        const double told = c->t;
         c->t = c->t + c->dt;
         if (salt && c->have_nbr) {
             for (int32_t k = 0; k < c->solsteps; ++k) {
@@ -1783,6 +1941,7 @@ int ora_step(ora_ctx* c, int32_t nsteps) {
                 if ((rc = ora_scalarupdate(c, c->dtsol))) return rc;
             }
         }
+        if (c->nzones && (rc = buffer_schedule(c, told, c->t))) return rc;
         if ((rc = ora_projection(c))) return rc;
         if ((rc = ora_cellshift(c))) return rc;
         if ((rc = ora_resetid(c))) return rc;

@@ -73,6 +73,10 @@ int ora_densityupdate(ora_ctx* c);
 int ora_salt_init(ora_ctx* c, double xl, double yl, double line_grad);
 int ora_get_substeps(ora_ctx* c, int32_t* solsteps, double* dtsol);
 int ora_probe(ora_ctx* c, int32_t np, const double* px, const double* py, double* out);
+int ora_set_buffer_zones(ora_ctx* c, int32_t nzones, const isph_buffer_zone* zones,
+                         double time_shift);
+int ora_remove_buffer(ora_ctx* c, int32_t zone);
+int ora_insert_buffer(ora_ctx* c, int32_t zone);
 
 /* Building blocks exposed for the oracle's own pinning tests. */
 double ora_kernel_w(const isph_params* prm, double r);
