@@ -25,6 +25,16 @@ module isph_capi
         integer(c_int32_t) :: shifting, adaptive_dt, sink, visc_law, salinity, cell_order, reserved
     end type isph_params
 
+    ! a buffer zone (include/isph_capi.h, isph_buffer_zone)
+    type, bind(C) :: isph_buffer_zone
+        integer(c_int32_t) :: ntemplate, ncells
+        type(c_ptr) :: x, y, vx, vy, pressure   ! real(c_double) arrays of ntemplate
+        type(c_ptr) :: flags                    ! integer(c_int32_t) array of ntemplate
+        type(c_ptr) :: cells                    ! integer(c_int32_t) (2, ncells)
+        real(c_double) :: mass, density, ovol, con, period, xcut
+        integer(c_int32_t) :: remove_right, reserved
+    end type isph_buffer_zone
+
     interface
         integer(c_int) function isph_create(prm, cap, cap_soil, ctx) bind(C, name="isph_create")
             import :: c_int, c_int32_t, c_ptr, isph_params
@@ -166,6 +176,25 @@ module isph_capi
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx
             real(c_double), value :: xl, yl, line_grad
+        end function
+        ! buffer zones (internal.f90:475-614, main.f90:103-121,218-236)
+        integer(c_int) function isph_set_buffer_zones(ctx, nzones, zones, time_shift) &
+                                                      bind(C, name="isph_set_buffer_zones")
+            import :: c_int, c_int32_t, c_ptr, c_double, isph_buffer_zone
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nzones
+            type(isph_buffer_zone), intent(in) :: zones(*)
+            real(c_double), value :: time_shift
+        end function
+        integer(c_int) function isph_remove_buffer(ctx, zone) bind(C, name="isph_remove_buffer")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: zone
+        end function
+        integer(c_int) function isph_insert_buffer(ctx, zone) bind(C, name="isph_insert_buffer")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: zone
         end function
         integer(c_int) function isph_get_substeps(ctx, solsteps, dtsol) &
                                                   bind(C, name="isph_get_substeps")

@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--salinity-only", action="store_true",
                     help="run only the salt-water step leg (for profiling)")
     ap.add_argument("--salt-dx", type=float, default=0.001)
+    ap.add_argument("--no-reference-run", action="store_true")
+    ap.add_argument("--reference-run-only", action="store_true")
     ap.add_argument("--adjoint-dx", type=float, default=0.002)
     ap.add_argument("--adjoint-steps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=400,
@@ -202,6 +204,59 @@ def run_scalart_at_scale(cfg, reps=20):
             "achieved_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def run_reference_scenario(steps, cpu_steps):
+    """The reference's own run (setup.f90, boundary.f90, internal.f90,
+    main.f90): its geometry (14,193 fluid, 1,233 wall, 21,795 soil
+    particles, 4h cells in the reference particle order, hybrid kernel,
+    reference drag, adaptive dt), `steps` steps of each of main.f90's three
+    phases (start-up, buffered, salt-water with transport), on the GPU and
+    `cpu_steps` per phase on the oracle (OpenMP, this machine's cores)."""
+    import torch
+    from beach_slope_confined_amd import capi, scenario
+    sc = scenario.reference_scenario()
+    cap = len(sc["parts"]["x"]) + 4000
+
+    def timed(sim, k):
+        sim.set_particles(sc["parts"])
+        sim.set_soil(sc["soil"])
+        marks = []
+
+        def hook(sim_, name):
+            if isinstance(sim_, capi.IsphSim):
+                torch.cuda.synchronize()
+            marks.append((name, time.perf_counter()))
+        if isinstance(sim, capi.IsphSim):
+            torch.cuda.synchronize()
+        marks.append(("init", time.perf_counter()))
+        scenario.run_reference(sim, sc, steps1=k, steps2=k, steps3=k, on_phase=hook)
+        out, last = {}, marks[0][1]
+        for name, tm in marks[1:]:
+            out[name] = round(1e3 * (tm - last) / k, 3)
+            last = tm
+        return out, sim.scalars()["iters"]
+    gpu = capi.IsphSim(sc["params"], cap, len(sc["soil"]["x"]))
+    timed(gpu, 2)                      # warm-up
+    gpu.close()
+    gpu = capi.IsphSim(sc["params"], cap, len(sc["soil"]["x"]))
+    g_ms, g_it = timed(gpu, steps)
+    gpu.close()
+    out = {"workload": "reference scenario (main.f90 phases 1-3, buffers, salinity): "
+                       f"{sc['n_fluid']} fluid, {len(sc['parts']['x'])} particles, "
+                       f"{len(sc['soil']['x'])} soil", "steps_per_phase": steps,
+           "gpu_ms_per_step": g_ms, "gpu_ppe_iters_last_step": g_it}
+    if cpu_steps:
+        from oracle import pyoracle
+        from oracle.pyoracle import OracleSim
+        cores = os.cpu_count() or 1
+        pyoracle.set_threads(min(16, cores))
+        ora = OracleSim(sc["params"], cap, len(sc["soil"]["x"]))
+        c_ms, _ = timed(ora, cpu_steps)
+        ora.close()
+        out.update(oracle_ms_per_step=c_ms, oracle_threads=min(16, cores),
+                   oracle_steps_per_phase=cpu_steps)
+    return out
+
+
 def run_adjoint(dx, nsteps, every):
     """Adjoint/forward cost on the porous beach-slope dam break: nsteps FP64
     steps with checkpoints every `every` steps, objective = time-mean
@@ -310,6 +365,10 @@ def main():
         return
     n_fluid = cfg["n_fluid"]
     n_total = len(cfg["parts"]["x"])
+    if args.reference_run_only:
+        print(json.dumps({"reference_run": run_reference_scenario(20, 0 if args.no_cpu else 2)}),
+              flush=True)
+        return
     if args.salinity_only:
         sal = run_salinity(args.salt_dx, args.steps, args.warmup)
         sal["k_scalart_tank"] = run_scalart_at_scale(cfg)
@@ -397,6 +456,12 @@ def main():
             line["salinity"]["k_scalart_tank"] = run_scalart_at_scale(cfg)
         except Exception as e:  # reported, not hidden
             line["salinity"] = {"error": str(e)}
+
+    if not args.no_reference_run:
+        try:
+            line["reference_run"] = run_reference_scenario(20, 0 if args.no_cpu else 2)
+        except Exception as e:  # reported, not hidden
+            line["reference_run"] = {"error": str(e)}
 
     if not args.no_adjoint:
         try:
