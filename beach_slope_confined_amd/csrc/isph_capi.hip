@@ -237,13 +237,13 @@ int run_assemble(isph_ctx* c) {
     if (c->n)
         k_assemble<<<nblk(c->n), BLOCK, 0, c->stream>>>(
             c->n, kconsts(c), p.lam, p.rho0 * p.rel_den, (size_t)c->cap, c->st[c->cur], c->nbr,
-            c->cnt, c->L, c->val64,
-            (p.precision == ISPH_PREC_MIXED || p.precision == ISPH_PREC_MIXED_V64) ? c->val32 : nullptr,
-            p.precision == ISPH_PREC_MIXED_V64 ? c->rsum : nullptr, c->dpos,
-            c->rhs, c->ss, c->err);
+            c->cnt, c->L, c->val64, p.precision != ISPH_PREC_FP64 ? c->val32 : nullptr,
+            (p.precision == ISPH_PREC_MIXED_V64 || p.precision == ISPH_PREC_MIXED_MG) ? c->rsum
+                                                                                     : nullptr,
+            c->dpos, c->rhs, c->ss, c->err);
     HIPCHK(c, hipGetLastError());
     c->assembled = true;
-    c->assembled32 = p.precision == ISPH_PREC_MIXED || p.precision == ISPH_PREC_MIXED_V64;
+    c->assembled32 = p.precision != ISPH_PREC_FP64;
     return ISPH_OK;
 }
 
@@ -378,8 +378,7 @@ SolveCtl solve_ctl(const isph_ctx* c, int mixed) {
     return t;
 }
 
-ResidArgs resid_args(isph_ctx* c, double* x, double* r, const double* val = nullptr,
-                     const double* rhs = nullptr) {
+ResidArgs resid_args(isph_ctx* c, double* x, double* r, const double* val, const double* rhs) {
     ResidArgs a;
     a.n = c->n;
     a.ld = (size_t)c->cap;
@@ -588,14 +587,16 @@ int solve_mixed(isph_ctx* c) {
 
 int run_solve(isph_ctx* c) {
     if (!c->assembled) return fail(c, ISPH_E_STATE, "solve before assembly");
-    const bool mixed = c->prm.precision == ISPH_PREC_MIXED || c->prm.precision == ISPH_PREC_MIXED_V64;
+    const bool mixed = c->prm.precision != ISPH_PREC_FP64;
     if (mixed && !c->assembled32)
         return fail(c, ISPH_E_STATE, "mixed precision needs an assembly with a mixed precision mode");
     StateView& s = c->st[c->cur];
     double* x = c->v64[8];
     if (c->n == 0) return ISPH_OK;
     HIPCHK(c, hipMemcpyAsync(x, s.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
-    int rc = mixed ? solve_mixed(c) : solve_fp64(c);
+    int rc = c->prm.precision == ISPH_PREC_MIXED_MG ? solve_mg(c)
+             : mixed                                ? solve_mixed(c)
+                                                    : solve_fp64(c);
     int rc2 = read_sc(c);
     if (rc == ISPH_OK) rc = rc2;
     c->iters = c->sc_host->total;
@@ -768,6 +769,7 @@ int isph_destroy(isph_ctx* c) {
     if (c->ss_host) (void)hipHostFree(c->ss_host);
     adj_free(c);
     free_zones(c);
+    mg_free(c);
     for (auto& sl : c->slots) {
         if (!sl.v.pos) continue;
         free_state(sl.v);

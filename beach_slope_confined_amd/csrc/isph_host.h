@@ -35,6 +35,7 @@ struct Profiler {
 };
 
 struct AdjointState;  // isph_adjoint.hip
+struct MGData;        // isph_mg.hip
 
 // a buffer zone as installed by isph_set_buffer_zones (isph_buffers.hip)
 struct BufferZone {
@@ -99,6 +100,7 @@ struct isph_ctx {
         bool used = false;
     } slots[64];
     isph::host::AdjointState* adj = nullptr;
+    isph::host::MGData* mg = nullptr;
     isph::host::BufferZone zones[ISPH_MAX_ZONES];
     int nzones = 0;
     double zone_shift = 0.0;
@@ -161,6 +163,14 @@ int run_step(isph_ctx* c);
 // (unit diagonal at dpos) and right-hand side; x holds the initial guess
 // on entry and the solution on return.
 int solve_fp64_on(isph_ctx* c, const double* val, const double* rhs, double* x);
+
+// solver pieces shared with the multigrid solve (isph_mg.hip)
+__global__ void k_fill(int n, double* x, double v);
+SolveCtl solve_ctl(const isph_ctx* c, int mixed);
+ResidArgs resid_args(isph_ctx* c, double* x, double* r, const double* val = nullptr,
+                     const double* rhs = nullptr);
+int solve_mg(isph_ctx* c);
+void mg_free(isph_ctx* c);
 
 // discrete adjoint (isph_adjoint.hip)
 void adj_free(isph_ctx* c);

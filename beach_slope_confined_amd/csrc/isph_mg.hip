@@ -1,0 +1,603 @@
+// isph_mg.hip -- multigrid-preconditioned mixed-precision PPE solve
+// (ISPH_PREC_MIXED_MG).
+//
+// BiCGSTAB with right preconditioning by one V-cycle of an aggregation
+// multigrid built on the particle cell grid:
+//   level 0  the PPE rows (particles), FP32 values of the assembled matrix;
+//   level 1  the cells: aggregate = the particles of a cell, piecewise
+//            constant prolongation, Galerkin operator A1 = P^T A P, which is
+//            a 9-point stencil on the cell grid because every neighbour of a
+//            particle lies in the 3x3 cells around its own;
+//   level l+1  2x2 blocks of level-l cells, again Galerkin and 9-point;
+//   coarsest  at most MG_CMAX cells, damped Jacobi sweeps in one block.
+// Smoothing is one damped-Jacobi sweep before (from x = 0) and one after
+// the coarse correction.  The V-cycle is a fixed linear operator, so the
+// preconditioned recurrences are the ordinary ones:
+//   p = r + beta (p - omega v),  p^ = M p,  v = A p^,  alpha = rho / (rhat, v)
+//   s = r - alpha v,             s^ = M s,  t = A s^,  omega = (t, s)/(t, t)
+//   x += alpha p^ + omega s^,    r = s - omega t
+// The outer vectors and all dot products are FP64; A p^ and A s^ use the
+// FP32 values in the conservation form with FP64 row sums (as the
+// MIXED_V64 solve); x accumulates in a segment flushed into the FP64
+// solution by reliable updates against the FP64 true residual, which is
+// also the stop test (||b - A64 x|| <= tol ||b||).  The scalars follow the
+// same protocol as the unpreconditioned solves and are finished by
+// k_finish.  Only the preconditioner is FP32.
+//
+// The hierarchy is rebuilt for every solve (the matrix changes every step):
+// two kernels per level, one thread per coarse cell summing its children in
+// a fixed order, so the setup is deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "isph_host.h"
+#include "isph_kernels.h"
+
+namespace isph {
+
+constexpr int MG_CMAX = 1024;        // coarsest level: at most this many cells
+constexpr int MG_COARSE_SWEEPS = 40;  // damped Jacobi sweeps on the coarsest level
+constexpr float MG_OMEGA = 0.7f;      // Jacobi damping
+
+__device__ __forceinline__ int mg_col(int e, int) { return e; }
+__device__ __forceinline__ int mg_col(short e, int i) { return i + (int)e; }
+
+// cell of every particle on the level-1 grid (cx * ny + cy of its bin)
+__global__ void k_mg_cells(int n, GridInfo g, const int* __restrict__ key, int* __restrict__ cell) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    int cx, cy;
+    key_cell(g, key[i], cx, cy);
+    cell[i] = cx * g.ny + cy;
+}
+
+// A1 = P^T A P: per cell, its rows' entries summed by the neighbour's cell
+// offset; the unit diagonal adds 1 per row.  An entry outside the 3x3 cells
+// raises ERR_DOMAIN (cannot happen for lists built from the cell stencil).
+__global__ void k_mg_galerkin1(GridInfo g, const int* __restrict__ bstart, size_t ld,
+                               const int* __restrict__ nbr, const int* __restrict__ cnt,
+                               const double* __restrict__ val, const int* __restrict__ cell,
+                               float* __restrict__ A1, uint32_t* err) {
+    const int nc = g.nx * g.ny;
+    int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= nc) return;
+    const int cx = c / g.ny, cy = c - cx * g.ny;
+    int lo, hi;
+    column_bins(g, cx, cy, cy, lo, hi);
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = bstart[lo]; i < bstart[hi]; ++i) {
+        acc[4] += 1.0;
+        const int k1 = cnt[i];
+        for (int k = 0; k < k1; ++k) {
+            const size_t o = (size_t)k * ld + i;
+            const int cj = cell[nbr[o]];
+            const int jx = cj / g.ny, jy = cj - jx * g.ny;
+            const int dx = jx - cx, dy = jy - cy;
+            if (dx < -1 || dx > 1 || dy < -1 || dy > 1) {
+                atomicOr(err, ERR_DOMAIN);
+                continue;
+            }
+            acc[(dx + 1) * 3 + (dy + 1)] += val[o];
+        }
+    }
+    for (int d = 0; d < 9; ++d) A1[(size_t)d * nc + c] = (float)acc[d];
+}
+
+// A_{l+1} from A_l: coarse cell C = (X, Y) owns the fine cells (2X+a, 2Y+b)
+__global__ void k_mg_galerkin(int nx, int ny, const float* __restrict__ A, int nx2, int ny2,
+                              float* __restrict__ A2) {
+    const int nc = nx * ny, nc2 = nx2 * ny2;
+    int C = blockIdx.x * BLOCK + threadIdx.x;
+    if (C >= nc2) return;
+    const int X = C / ny2, Y = C - X * ny2;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const int cx = 2 * X + a, cy = 2 * Y + b;
+            if (cx >= nx || cy >= ny) continue;
+            const int c = cx * ny + cy;
+            for (int d = 0; d < 9; ++d) {
+                const int ex = cx + d / 3 - 1, ey = cy + d % 3 - 1;
+                if (ex < 0 || ex >= nx || ey < 0 || ey >= ny) continue;
+                const int D = ((ex >> 1) - X + 1) * 3 + ((ey >> 1) - Y + 1);
+                acc[D] += (double)A[(size_t)d * nc + c];
+            }
+        }
+    for (int d = 0; d < 9; ++d) A2[(size_t)d * nc2 + C] = (float)acc[d];
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle kernels (FP32)
+// ---------------------------------------------------------------------------
+// Level 0 down.  The right-hand side b is formed from the outer pairs:
+// MODE 0: b = p = r + beta q (r on a restart), from rq = (r, q);
+// MODE 1: b = s = r - alpha v, from rv = (r, v).  b goes to bout (FP64,
+// the outer recurrences need it); x0 = omega b (Jacobi from 0, unit
+// diagonal) and r0 = b - A x0 = b - omega A b.
+template <class TI, int MODE, bool FIRST>
+__global__ void __launch_bounds__(BLOCK) k_mg_fine_down(int n, size_t ld, const TI* __restrict__ nbr,
+                                                        const int* __restrict__ cnt,
+                                                        const float* __restrict__ val,
+                                                        const double2* __restrict__ pairs,
+                                                        const SolverScalars* sc,
+                                                        double* __restrict__ bout,
+                                                        float* __restrict__ x0,
+                                                        float* __restrict__ r0) {
+    if (sc->done) return;
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double coef = MODE == 0 ? sc->betas : sc->alpha;
+    auto bj = [&](int j) -> double {
+        const double2 u = pairs[j];
+        if (MODE == 0) return FIRST ? u.x : u.x + coef * u.y;
+        return u.x - coef * u.y;
+    };
+    const double bi = bj(i);
+    bout[i] = bi;
+    const float fb = (float)bi;
+    float acc = fb;
+    const int c = cnt[i];
+    for (int k = 0; k < c; ++k) {
+        const size_t o = (size_t)k * ld + i;
+        acc += val[o] * (float)bj(mg_col(nbr[o], i));
+    }
+    x0[i] = MG_OMEGA * fb;
+    r0[i] = fb - MG_OMEGA * acc;
+}
+
+// restriction to the cells: b1[c] = sum of r0 over the cell's particles
+__global__ void k_mg_restrict1(GridInfo g, const int* __restrict__ bstart,
+                               const float* __restrict__ r0, float* __restrict__ b1,
+                               const SolverScalars* sc) {
+    if (sc->done) return;
+    const int nc = g.nx * g.ny;
+    int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= nc) return;
+    const int cx = c / g.ny, cy = c - cx * g.ny;
+    int lo, hi;
+    column_bins(g, cx, cy, cy, lo, hi);
+    float s = 0.0f;
+    for (int i = bstart[lo]; i < bstart[hi]; ++i) s += r0[i];
+    b1[c] = s;
+}
+
+__device__ __forceinline__ float mg_jac0(const float* __restrict__ A, int nc, int c,
+                                         const float* __restrict__ b) {
+    const float dg = A[(size_t)4 * nc + c];
+    return dg != 0.0f ? MG_OMEGA * b[c] / dg : 0.0f;
+}
+
+// grid level down: x = omega D^-1 b, r = b - A x
+__global__ void k_mg_grid_down(int nx, int ny, const float* __restrict__ A,
+                               const float* __restrict__ b, float* __restrict__ x,
+                               float* __restrict__ r, const SolverScalars* sc) {
+    if (sc->done) return;
+    const int nc = nx * ny;
+    int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= nc) return;
+    const int cx = c / ny, cy = c - cx * ny;
+    float acc = 0.0f;
+    for (int d = 0; d < 9; ++d) {
+        const int ex = cx + d / 3 - 1, ey = cy + d % 3 - 1;
+        if (ex < 0 || ex >= nx || ey < 0 || ey >= ny) continue;
+        acc += A[(size_t)d * nc + c] * mg_jac0(A, nc, ex * ny + ey, b);
+    }
+    x[c] = mg_jac0(A, nc, c, b);
+    r[c] = b[c] - acc;
+}
+
+__global__ void k_mg_restrict(int nx, int ny, const float* __restrict__ r, int nx2, int ny2,
+                              float* __restrict__ b2, const SolverScalars* sc) {
+    if (sc->done) return;
+    int C = blockIdx.x * BLOCK + threadIdx.x;
+    if (C >= nx2 * ny2) return;
+    const int X = C / ny2, Y = C - X * ny2;
+    float s = 0.0f;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const int cx = 2 * X + a, cy = 2 * Y + b;
+            if (cx < nx && cy < ny) s += r[cx * ny + cy];
+        }
+    b2[C] = s;
+}
+
+// coarsest level: damped Jacobi sweeps from 0, one block, in LDS
+__global__ void __launch_bounds__(1024) k_mg_coarse(int nx, int ny, const float* __restrict__ A,
+                                                    const float* __restrict__ b,
+                                                    float* __restrict__ x, int sweeps,
+                                                    const SolverScalars* sc) {
+    if (sc->done) return;
+    __shared__ float xa[MG_CMAX], xb[MG_CMAX];
+    const int nc = nx * ny;
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) xa[c] = 0.0f;
+    __syncthreads();
+    float* cur = xa;
+    float* nxt = xb;
+    for (int it = 0; it < sweeps; ++it) {
+        for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+            const int cx = c / ny, cy = c - cx * ny;
+            float acc = 0.0f;
+            for (int d = 0; d < 9; ++d) {
+                const int ex = cx + d / 3 - 1, ey = cy + d % 3 - 1;
+                if (ex < 0 || ex >= nx || ey < 0 || ey >= ny) continue;
+                acc += A[(size_t)d * nc + c] * cur[ex * ny + ey];
+            }
+            const float dg = A[(size_t)4 * nc + c];
+            nxt[c] = dg != 0.0f ? cur[c] + MG_OMEGA * (b[c] - acc) / dg : 0.0f;
+        }
+        __syncthreads();
+        float* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) x[c] = cur[c];
+}
+
+// grid level up: z = x + e(parent), out = z + omega D^-1 (b - A z)
+__global__ void k_mg_grid_up(int nx, int ny, const float* __restrict__ A,
+                             const float* __restrict__ b, const float* __restrict__ x, int ny2,
+                             const float* __restrict__ e, float* __restrict__ out,
+                             const SolverScalars* sc) {
+    if (sc->done) return;
+    const int nc = nx * ny;
+    int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= nc) return;
+    const int cx = c / ny, cy = c - cx * ny;
+    auto z = [&](int ex, int ey) { return x[ex * ny + ey] + e[(ex >> 1) * ny2 + (ey >> 1)]; };
+    float acc = 0.0f;
+    for (int d = 0; d < 9; ++d) {
+        const int ex = cx + d / 3 - 1, ey = cy + d % 3 - 1;
+        if (ex < 0 || ex >= nx || ey < 0 || ey >= ny) continue;
+        acc += A[(size_t)d * nc + c] * z(ex, ey);
+    }
+    const float dg = A[(size_t)4 * nc + c];
+    const float zc = z(cx, cy);
+    out[c] = dg != 0.0f ? zc + MG_OMEGA * (b[c] - acc) / dg : 0.0f;
+}
+
+// level 0 up: z = x0 + e1(cell), out = z + omega (b - A z) in FP64
+template <class TI>
+__global__ void __launch_bounds__(BLOCK) k_mg_fine_up(int n, size_t ld, const TI* __restrict__ nbr,
+                                                      const int* __restrict__ cnt,
+                                                      const float* __restrict__ val,
+                                                      const double* __restrict__ b,
+                                                      const float* __restrict__ x0,
+                                                      const int* __restrict__ cell,
+                                                      const float* __restrict__ e1,
+                                                      double* __restrict__ out,
+                                                      const SolverScalars* sc) {
+    if (sc->done) return;
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    auto z = [&](int j) { return x0[j] + e1[cell[j]]; };
+    const float zi = z(i);
+    float acc = zi;
+    const int c = cnt[i];
+    for (int k = 0; k < c; ++k) {
+        const size_t o = (size_t)k * ld + i;
+        acc += val[o] * z(mg_col(nbr[o], i));
+    }
+    out[i] = (double)(zi + MG_OMEGA * ((float)b[i] - acc));
+}
+
+// ---------------------------------------------------------------------------
+// outer BiCGSTAB kernels (FP64 vectors, partials per XCD-mapped block)
+// ---------------------------------------------------------------------------
+template <class TI>
+__device__ __forceinline__ double mg_spmv(int i, int c, double si, const TI* __restrict__ nbr,
+                                          const float* __restrict__ val, size_t ld,
+                                          const double* __restrict__ x) {
+    const double xi = x[i];
+    double acc = 0.0;
+    for (int k = 0; k < c; ++k) {
+        const size_t o = (size_t)k * ld + i;
+        acc = acc + (double)val[o] * (x[mg_col(nbr[o], i)] - xi);
+    }
+    return si * xi + acc;
+}
+
+// v = A p^; rv = (r, v); partial (v, rhat)
+template <class TI>
+__global__ void __launch_bounds__(BLOCK) k_mg_v(int n, size_t ld, const TI* __restrict__ nbr,
+                                                const int* __restrict__ cnt,
+                                                const float* __restrict__ val,
+                                                const double* __restrict__ rsum,
+                                                const double* __restrict__ ph,
+                                                const double2* __restrict__ rq,
+                                                const double* __restrict__ rhat,
+                                                double2* __restrict__ rv, double* __restrict__ part,
+                                                const SolverScalars* sc) {
+    if (sc->done) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const double s = mg_spmv<TI>(i, cnt[i], rsum[i], nbr, val, ld, ph);
+        rv[i] = make_double2(rq[i].x, s);
+        v = s * rhat[i];
+    }
+    v = block_sum(v);
+    if (threadIdx.x == 0) part[lb] = v;
+}
+
+// t = A s^; partials (t, s), (t, t)
+template <class TI>
+__global__ void __launch_bounds__(BLOCK) k_mg_t(int n, size_t ld, const TI* __restrict__ nbr,
+                                                const int* __restrict__ cnt,
+                                                const float* __restrict__ val,
+                                                const double* __restrict__ rsum,
+                                                const double* __restrict__ sh,
+                                                const double* __restrict__ s,
+                                                double* __restrict__ t, double* __restrict__ part,
+                                                const SolverScalars* sc) {
+    if (sc->done) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    double v0 = 0.0, v1 = 0.0;
+    if (i < n) {
+        const double ti = mg_spmv<TI>(i, cnt[i], rsum[i], nbr, val, ld, sh);
+        t[i] = ti;
+        v0 = ti * s[i];
+        v1 = ti * ti;
+    }
+    v0 = block_sum(v0);
+    v1 = block_sum(v1);
+    if (threadIdx.x == 0) {
+        part[lb] = v0;
+        part[gridDim.x + lb] = v1;
+    }
+}
+
+// xs += alpha p^ + omega s^; r = s - omega t; q = p - omega v; partials (r, rhat), (r, r)
+__global__ void __launch_bounds__(BLOCK) k_mg_x(int n, const double* __restrict__ p,
+                                                const double* __restrict__ ph,
+                                                const double* __restrict__ s,
+                                                const double* __restrict__ sh,
+                                                const double* __restrict__ t,
+                                                const double2* __restrict__ rv,
+                                                const double* __restrict__ rhat,
+                                                double* __restrict__ xs, double2* __restrict__ rq,
+                                                double* __restrict__ part, const SolverScalars* sc) {
+    if (sc->done) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * BLOCK + threadIdx.x;
+    double v0 = 0.0, v1 = 0.0;
+    if (i < n) {
+        const double al = sc->alpha, w = sc->wi;
+        xs[i] = (xs[i] + al * ph[i]) + w * sh[i];
+        const double r = s[i] - w * t[i];
+        const double q = p[i] - w * rv[i].y;
+        rq[i] = make_double2(r, q);
+        v0 = r * rhat[i];
+        v1 = r * r;
+    }
+    v0 = block_sum(v0);
+    v1 = block_sum(v1);
+    if (threadIdx.x == 0) {
+        part[lb] = v0;
+        part[gridDim.x + lb] = v1;
+    }
+}
+
+namespace host {
+
+struct MGLevel {
+    int nx = 0, ny = 0;
+    float *A = nullptr, *b = nullptr, *x = nullptr, *r = nullptr, *x2 = nullptr;
+};
+
+struct MGData {
+    int nlev = 0;  // grid levels (level 1 .. nlev); the last is the coarsest
+    MGLevel lv[24];
+    float *x0 = nullptr, *r0 = nullptr;
+    int* cell = nullptr;
+};
+
+void mg_free(isph_ctx* c) {
+    MGData* m = c->mg;
+    if (!m) return;
+    for (int l = 0; l < m->nlev; ++l) {
+        void* p[] = {m->lv[l].A, m->lv[l].b, m->lv[l].x, m->lv[l].r, m->lv[l].x2};
+        for (auto q : p)
+            if (q) (void)hipFree(q);
+    }
+    void* p[] = {m->x0, m->r0, m->cell};
+    for (auto q : p)
+        if (q) (void)hipFree(q);
+    delete m;
+    c->mg = nullptr;
+}
+
+static int mg_alloc(isph_ctx* c) {
+    if (c->mg) return ISPH_OK;
+    MGData* m = new MGData();
+    c->mg = m;
+    int nx = c->grid.nx, ny = c->grid.ny;
+    int rc = 0;
+    for (;;) {
+        if (m->nlev >= 24) break;
+        MGLevel& L = m->lv[m->nlev++];
+        L.nx = nx;
+        L.ny = ny;
+        const size_t nc = (size_t)nx * ny;
+        rc |= dalloc(c, &L.A, 9 * nc);
+        rc |= dalloc(c, &L.b, nc);
+        rc |= dalloc(c, &L.x, nc);
+        rc |= dalloc(c, &L.r, nc);
+        rc |= dalloc(c, &L.x2, nc);
+        if ((int64_t)nx * ny <= MG_CMAX) break;
+        nx = (nx + 1) / 2;
+        ny = (ny + 1) / 2;
+    }
+    const size_t N = (size_t)c->cap;
+    rc |= dalloc(c, &m->x0, N);
+    rc |= dalloc(c, &m->r0, N);
+    rc |= dalloc(c, &m->cell, N);
+    if (rc || (int64_t)m->lv[m->nlev - 1].nx * m->lv[m->nlev - 1].ny > MG_CMAX) {
+        mg_free(c);
+        return fail(c, ISPH_E_HIP, "multigrid hierarchy allocation failed");
+    }
+    return ISPH_OK;
+}
+
+// Galerkin hierarchy of the current matrix
+static int mg_setup(isph_ctx* c) {
+    int rc = mg_alloc(c);
+    if (rc) return rc;
+    MGData* m = c->mg;
+    const GridInfo& g = c->grid;
+    k_mg_cells<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, g, c->key_s, m->cell);
+    k_mg_galerkin1<<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
+        g, c->bincnt, (size_t)c->cap, c->nbr, c->cnt, c->val64, m->cell, m->lv[0].A, c->err);
+    for (int l = 0; l + 1 < m->nlev; ++l) {
+        const MGLevel &a = m->lv[l], &b = m->lv[l + 1];
+        k_mg_galerkin<<<nblk((int64_t)b.nx * b.ny), BLOCK, 0, c->stream>>>(a.nx, a.ny, a.A, b.nx,
+                                                                            b.ny, b.A);
+    }
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+// one V-cycle: b formed from `pairs` (MODE 0: p from rq, MODE 1: s from rv)
+// into bout, preconditioned vector into out
+template <class TI, int MODE, bool FIRST>
+static void mg_vcycle(isph_ctx* c, const TI* nbr, const double2* pairs, double* bout, double* out) {
+    MGData* m = c->mg;
+    const GridInfo& g = c->grid;
+    const int n = c->n;
+    const size_t ld = (size_t)c->cap;
+    k_mg_fine_down<TI, MODE, FIRST><<<nblk(n), BLOCK, 0, c->stream>>>(
+        n, ld, nbr, c->cnt, c->val32, pairs, c->sc, bout, m->x0, m->r0);
+    k_mg_restrict1<<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(g, c->bincnt, m->r0,
+                                                                         m->lv[0].b, c->sc);
+    const int L = m->nlev;
+    for (int l = 0; l + 1 < L; ++l) {
+        MGLevel &a = m->lv[l], &b = m->lv[l + 1];
+        k_mg_grid_down<<<nblk((int64_t)a.nx * a.ny), BLOCK, 0, c->stream>>>(a.nx, a.ny, a.A, a.b,
+                                                                             a.x, a.r, c->sc);
+        k_mg_restrict<<<nblk((int64_t)b.nx * b.ny), BLOCK, 0, c->stream>>>(a.nx, a.ny, a.r, b.nx,
+                                                                            b.ny, b.b, c->sc);
+    }
+    MGLevel& cl = m->lv[L - 1];
+    k_mg_coarse<<<1, 1024, 0, c->stream>>>(cl.nx, cl.ny, cl.A, cl.b, cl.x, MG_COARSE_SWEEPS, c->sc);
+    for (int l = L - 2; l >= 0; --l) {
+        MGLevel &a = m->lv[l], &b = m->lv[l + 1];
+        k_mg_grid_up<<<nblk((int64_t)a.nx * a.ny), BLOCK, 0, c->stream>>>(a.nx, a.ny, a.A, a.b, a.x,
+                                                                           b.ny, b.x, a.x2, c->sc);
+        std::swap(a.x, a.x2);
+    }
+    k_mg_fine_up<TI><<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, c->val32, bout, m->x0,
+                                                        m->cell, m->lv[0].x, out, c->sc);
+}
+
+template <class TI>
+static int mg_iterate(isph_ctx* c, const TI* nbr, const SolveCtl& ctl, bool& first) {
+    const int grid = nblk8(c->n);
+    const int n = c->n;
+    const size_t ld = (size_t)c->cap;
+    double *rhat = c->v64[0], *p = c->v64[1], *s = c->v64[2], *t = c->v64[3], *xs = c->v64[4];
+    double *ph = c->v64[5], *sh = c->v64[6];
+    for (;;) {
+        const int before = c->sc_host->total;
+        (void)before;
+        int batch = 4;
+        const SolverScalars& h = *c->sc_host;
+        if (h.nb > 0 && h.rr > 0) {
+            double target = std::max(ctl.tol * h.nb, ctl.delta * h.rupd);
+            const double ratio = std::sqrt(h.rr) / target;
+            batch = ratio > 100.0 ? 8 : (ratio > 3.0 ? 4 : 2);
+        }
+        for (int b = 0; b < batch; ++b) {
+            if (first)
+                mg_vcycle<TI, 0, true>(c, nbr, c->rq64, p, ph);
+            else
+                mg_vcycle<TI, 0, false>(c, nbr, c->rq64, p, ph);
+            k_mg_v<TI><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, c->val32, c->rsum, ph,
+                                                       c->rq64, rhat, c->rv64, c->part, c->sc);
+            k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_ALPHA, grid, c->part, c->sc, ctl);
+            mg_vcycle<TI, 1, false>(c, nbr, c->rv64, s, sh);
+            k_mg_t<TI><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, c->val32, c->rsum, sh, s,
+                                                       t, c->part, c->sc);
+            k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_OMEGA, grid, c->part, c->sc, ctl);
+            k_mg_x<<<grid, BLOCK, 0, c->stream>>>(n, p, ph, s, sh, t, c->rv64, rhat, xs, c->rq64,
+                                                   c->part, c->sc);
+            k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_BETA, grid, c->part, c->sc, ctl);
+            first = false;
+        }
+        HIPCHK(c, hipGetLastError());
+        int rc = read_sc(c);
+        if (rc) return rc;
+        if (c->sc_host->done) return ISPH_OK;
+    }
+}
+
+template <class TI>
+static int solve_mg_t(isph_ctx* c, const TI* nbr) {
+    const int grid = nblk8(c->n);
+    double* x = c->v64[8];
+    double* r64 = c->v64[9];
+    int rc = mg_setup(c);
+    if (rc) return rc;
+    SolveCtl ctl = solve_ctl(c, 1);
+    SolveCtl ctl_flush = solve_ctl(c, 2);
+    ResidArgs ra = resid_args(c, x, r64);
+    k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 0, nullptr, nullptr);
+    k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, solve_ctl(c, 0));
+    k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_RESID, grid, c->part, c->sc, ctl);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = read_sc(c))) return rc;
+    if (c->sc_host->done == DONE_ZERO_RHS) {
+        k_fill<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, x, 0.0);
+        return ISPH_OK;
+    }
+    if (c->sc_host->done == DONE_CONVERGED) return ISPH_OK;
+    k_mx_start<double><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, c->rq64, c->v64[0], c->v64[4],
+                                                      c->part, c->sc);
+    k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_START, grid, c->part, c->sc, ctl);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = read_sc(c))) return rc;
+    bool first = true;
+    for (;;) {
+        if ((rc = mg_iterate<TI>(c, nbr, ctl, first))) return rc;
+        const int d = c->sc_host->done;
+        if (d == DONE_FAILED) return fail(c, ISPH_E_NOCONV, "multigrid PPE solve did not converge");
+        if (d != DONE_UPDATE) return fail(c, ISPH_E_STATE, "unexpected solver state");
+        k_mx_flush<double><<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, x, c->v64[4]);
+        k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 0, nullptr, nullptr);
+        k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_RESID, grid, c->part, c->sc, ctl_flush);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = read_sc(c))) return rc;
+        if (c->sc_host->done == DONE_CONVERGED) return ISPH_OK;
+        if (c->sc_host->done == DONE_FAILED)
+            return fail(c, ISPH_E_NOCONV, "multigrid PPE solve did not converge");
+        if (c->sc_host->brk) {
+            k_mx_start<double><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, c->rq64, c->v64[0],
+                                                              c->v64[4], c->part, c->sc);
+            k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_START, grid, c->part, c->sc, ctl);
+            first = true;
+        } else {
+            k_mx_replace<double><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, c->rq64, c->v64[0],
+                                                                c->part, c->sc);
+            k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_REPLACE, grid, c->part, c->sc, ctl);
+        }
+        HIPCHK(c, hipGetLastError());
+        if ((rc = read_sc(c))) return rc;
+    }
+}
+
+int solve_mg(isph_ctx* c) {
+    k_to_short<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, (size_t)c->cap, c->nbr, c->cnt, c->nbr16,
+                                                     c->err);
+    uint32_t e = 0;
+    HIPCHK(c, hipMemcpyAsync(&e, c->err, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const bool narrow = !(e & ERR_INDEX16);
+    if (!narrow) HIPCHK(c, hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->stream));
+    c->narrow_idx = narrow;
+    return narrow ? solve_mg_t<short>(c, c->nbr16) : solve_mg_t<int>(c, c->nbr);
+}
+
+}  // namespace host
+}  // namespace isph

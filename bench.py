@@ -49,6 +49,8 @@ def parse():
                     help="run only the salt-water step leg (for profiling)")
     ap.add_argument("--salt-dx", type=float, default=0.001)
     ap.add_argument("--no-reference-run", action="store_true")
+    ap.add_argument("--mg-only", action="store_true",
+                    help="time only the multigrid-preconditioned step (for profiling)")
     ap.add_argument("--reference-run-only", action="store_true")
     ap.add_argument("--adjoint-dx", type=float, default=0.002)
     ap.add_argument("--adjoint-steps", type=int, default=50)
@@ -75,11 +77,12 @@ def kernel_bytes(kind, nnz_off, n, val_bytes, vec_bytes, idx_bytes=4):
     return idx + val + vec
 
 
-def run_gpu(cfg, steps, warmup, precision, profile):
+def run_gpu(cfg, steps, warmup, precision, profile, tol=None):
     from beach_slope_confined_amd import capi, configs
     from beach_slope_confined_amd.params import PREC_FP64
     c = dict(cfg)
-    c["params"] = cfg["params"].copy(precision=precision)
+    c["params"] = cfg["params"].copy(precision=precision, tol=tol or cfg["params"].tol,
+                                     maxit=200000)
     sim = configs.make_sim(c, capi.IsphSim)
     # every step starts from the same device-resident input (the workload is
     # one ISPH step at scale; the snapshot copy is device-to-device)
@@ -114,6 +117,16 @@ def run_gpu(cfg, steps, warmup, precision, profile):
         k0, k1 = (0, 1) if precision == PREC_FP64 else (2, 3)
         out["ktime"] = {"p_spmv": sim.kernel_time(k0), "s_spmv": sim.kernel_time(k1)}
     return out
+
+
+def step_errors(a, b):
+    """max |p_b - p_a| / max |p_a| and the velocity error over the fluid,
+    relative to the largest fluid speed of a"""
+    f = a["tid"] == 3
+    perr = float(np.abs(b["pressure"] - a["pressure"]).max() / np.abs(a["pressure"]).max())
+    vmag = np.hypot(a["vx"][f], a["vy"][f]).max()
+    verr = float(max(np.abs(b["vx"] - a["vx"])[f].max(), np.abs(b["vy"] - a["vy"])[f].max()) / vmag)
+    return perr, verr
 
 
 def scalart_bytes(ptr, tid):
@@ -347,7 +360,7 @@ def main():
     import torch
     torch.cuda.init()
     from beach_slope_confined_amd import configs
-    from beach_slope_confined_amd.params import PREC_FP64, PREC_MIXED, PREC_MIXED_V64
+    from beach_slope_confined_amd.params import PREC_FP64, PREC_MIXED, PREC_MIXED_MG, PREC_MIXED_V64
 
     cfg = configs.tank_1m(args.nside)
     if args.pmc_run:
@@ -373,6 +386,15 @@ def main():
         sal = run_salinity(args.salt_dx, args.steps, args.warmup)
         sal["k_scalart_tank"] = run_scalart_at_scale(cfg)
         print(json.dumps({"salinity": sal}), flush=True)
+        return
+
+    if args.mg_only:
+        rmg = run_gpu(cfg, args.steps, args.warmup, PREC_MIXED_MG, False, tol=1e-9)
+        rmg["sim"].close()
+        print(json.dumps({"mixed_mg": {
+            "value": round(n_fluid * args.steps / rmg["seconds"], 1),
+            "ms_per_step": round(1e3 * rmg["seconds"] / args.steps, 3), "tol": 1e-9,
+            "iters_per_step": rmg["iters"][args.warmup:], "relres": rmg["relres"]}}), flush=True)
         return
 
     r64 = run_gpu(cfg, args.steps, args.warmup, PREC_FP64, True)
@@ -449,6 +471,37 @@ def main():
                                     "achieved_GBs": round(bm / avgm / 1e9, 1) if lm else None},
             }
             rmx["sim"].close()
+
+    if not args.no_mixed:
+        # multigrid-preconditioned mixed PPE, solved to 1e-9; errors against
+        # the FP64 step and against a tightly solved FP64 step (1e-12: FP64
+        # BiCGSTAB's recurrence stalls above 1e-13 at a million rows)
+        try:
+            rmg = run_gpu(cfg, args.steps, args.warmup, PREC_MIXED_MG, False, tol=1e-9)
+            rmg["sim"].close()
+            vmg = n_fluid * args.steps / rmg["seconds"]
+            pe, ve = step_errors(r64["first"], rmg["first"])
+            line["mixed_mg"] = {
+                "value": round(vmg, 1), "ms_per_step": round(1e3 * rmg["seconds"] / args.steps, 3),
+                "speedup_vs_fp64": round(vmg / value, 3), "tol": 1e-9,
+                "iters_per_step": rmg["iters"][args.warmup:], "reliable_updates": rmg["outer"],
+                "relres": rmg["relres"], "pressure_err_vs_fp64": pe, "velocity_err_vs_fp64": ve,
+            }
+        except Exception as e:  # reported, not hidden
+            line["mixed_mg"] = {"error": str(e)}
+        if "value" in line["mixed_mg"]:
+            try:
+                rtight = run_gpu(cfg, 0, 1, PREC_FP64, False, tol=1e-12)
+                rtight["sim"].close()
+                pt, vt = step_errors(rtight["first"], rmg["first"])
+                p64, v64e = step_errors(rtight["first"], r64["first"])
+                line["mixed_mg"].update({
+                    "tight_tol": 1e-12, "tight_iters": rtight["iters"][0],
+                    "pressure_err_vs_tight": pt, "velocity_err_vs_tight": vt,
+                    "fp64_pressure_err_vs_tight": p64, "fp64_velocity_err_vs_tight": v64e,
+                })
+            except Exception as e:
+                line["mixed_mg"]["tight_error"] = str(e)
 
     if not args.no_salinity:
         try:

@@ -61,6 +61,9 @@ extern "C" {
                                   dot products, FP64 residual + reliable updates     */
 #define ISPH_PREC_MIXED_V64 2  /* FP32 matrix values, FP64 Krylov vectors, FP64
                                   residual + reliable updates                        */
+#define ISPH_PREC_MIXED_MG  3  /* as MIXED_V64, right-preconditioned by one FP32
+                                  V-cycle of an aggregation multigrid on the cell
+                                  grid (cells, then 2x2 blocks of cells)             */
 
 /* Particle order kept by cellshift/cellshiftalt (isph_params.cell_order).
  * ISPH_ORDER_CELL: by cell (column-major, y ascending), walls first, then

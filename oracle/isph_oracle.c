@@ -1397,6 +1397,9 @@ int ora_ppe_solve(ora_ctx* c, int32_t* iters, double* relres) {
     int32_t it = 0, outer = 0;
     double rel = 0.0;
     int rc;
+    if (prm->precision == ISPH_PREC_MIXED_MG)
+        return set_err(c, ISPH_E_ARG, "the multigrid solve is checked against the FP64 solve, "
+                                      "it has no restatement here");
     if (prm->precision == ISPH_PREC_MIXED || prm->precision == ISPH_PREC_MIXED_V64) {
         rc = mixed_solve(c, x, &it, &outer, &rel);
     } else {
