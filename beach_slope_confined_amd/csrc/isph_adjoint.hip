@@ -476,6 +476,8 @@ int check_adjoint_params(isph_ctx* c) {
         return fail(c, ISPH_E_ARG, "the adjoint does not support salinity transport (salinity = 0)");
     if (c->prm.cell_order != ISPH_ORDER_CELL)
         return fail(c, ISPH_E_ARG, "the adjoint runs with cell_order = ISPH_ORDER_CELL");
+    if (c->prm.precision == ISPH_PREC_FGMRES)
+        return fail(c, ISPH_E_ARG, "the adjoint needs a BiCGSTAB forward solve (not FGMRES)");
     return ISPH_OK;
 }
 

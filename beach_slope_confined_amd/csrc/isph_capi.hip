@@ -43,12 +43,19 @@ GridInfo make_grid(const isph_params& p) {
     return g;
 }
 
+bool prec_mixed(int precision) {
+    return precision == ISPH_PREC_MIXED || precision == ISPH_PREC_MIXED_V64 ||
+           precision == ISPH_PREC_MIXED_MG;
+}
+
 int check_params(isph_ctx* c, const isph_params* p) {
     if (!(p->h > 0.0) || !(p->cell_size >= 2.0 * p->h) || !(p->xmax > p->xmin) ||
         !(p->ymax > p->ymin) || p->max_nbr <= 0)
         return fail(c, ISPH_E_ARG, "invalid geometry parameters");
     if (p->cell_order != ISPH_ORDER_CELL && p->cell_order != ISPH_ORDER_REFERENCE)
         return fail(c, ISPH_E_ARG, "unknown cell_order");
+    if (p->precision < ISPH_PREC_FP64 || p->precision > ISPH_PREC_FGMRES)
+        return fail(c, ISPH_E_ARG, "unknown precision");
     if (p->cell_order == ISPH_ORDER_REFERENCE && !(p->cell_size >= 4.0 * p->h))
         return fail(c, ISPH_E_ARG, "the reference order needs cell_size >= 4h (2x2-cell blocks)");
     GridInfo g = make_grid(*p);
@@ -237,13 +244,13 @@ int run_assemble(isph_ctx* c) {
     if (c->n)
         k_assemble<<<nblk(c->n), BLOCK, 0, c->stream>>>(
             c->n, kconsts(c), p.lam, p.rho0 * p.rel_den, (size_t)c->cap, c->st[c->cur], c->nbr,
-            c->cnt, c->L, c->val64, p.precision != ISPH_PREC_FP64 ? c->val32 : nullptr,
+            c->cnt, c->L, c->val64, prec_mixed(p.precision) ? c->val32 : nullptr,
             (p.precision == ISPH_PREC_MIXED_V64 || p.precision == ISPH_PREC_MIXED_MG) ? c->rsum
                                                                                      : nullptr,
             c->dpos, c->rhs, c->ss, c->err);
     HIPCHK(c, hipGetLastError());
     c->assembled = true;
-    c->assembled32 = p.precision != ISPH_PREC_FP64;
+    c->assembled32 = prec_mixed(p.precision);
     return ISPH_OK;
 }
 
@@ -587,12 +594,26 @@ int solve_mixed(isph_ctx* c) {
 
 int run_solve(isph_ctx* c) {
     if (!c->assembled) return fail(c, ISPH_E_STATE, "solve before assembly");
-    const bool mixed = c->prm.precision != ISPH_PREC_FP64;
+    const bool mixed = prec_mixed(c->prm.precision);
     if (mixed && !c->assembled32)
         return fail(c, ISPH_E_STATE, "mixed precision needs an assembly with a mixed precision mode");
     StateView& s = c->st[c->cur];
     double* x = c->v64[8];
     if (c->n == 0) return ISPH_OK;
+    if (c->prm.precision == ISPH_PREC_FGMRES) {
+        // warm start fsol in row order (solver.f90:319); the solution is
+        // fetched into fsol only on convergence (solver.f90:330-333), and
+        // the pressure is fsol in either case (isph.f90:465-483)
+        const size_t nd = sizeof(double) * c->n;
+        HIPCHK(c, hipMemcpyAsync(x, c->fsol, nd, hipMemcpyDeviceToDevice, c->stream));
+        bool conv = false;
+        int rc = solve_gmres(c, x, conv);
+        c->outer = 0;
+        if (rc) return rc;
+        if (conv) HIPCHK(c, hipMemcpyAsync(c->fsol, x, nd, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(s.p, c->fsol, nd, hipMemcpyDeviceToDevice, c->stream));
+        return ISPH_OK;
+    }
     HIPCHK(c, hipMemcpyAsync(x, s.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     int rc = c->prm.precision == ISPH_PREC_MIXED_MG ? solve_mg(c)
              : mixed                                ? solve_mixed(c)
@@ -730,6 +751,7 @@ int isph_create(const isph_params* prm, int32_t cap, int32_t cap_soil, isph_ctx*
     rc |= dalloc(c, &c->pos_nb, N);
     rc |= dalloc(c, &c->cdiff, N);
     rc |= dalloc(c, &c->salt_coef, N);
+    rc |= dalloc(c, &c->fsol, N);
     if (hipHostMalloc((void**)&c->sc_host, sizeof(SolverScalars), hipHostMallocDefault) != hipSuccess)
         rc = ISPH_E_HIP;
     if (hipHostMalloc((void**)&c->ss_host, sizeof(SimScalars), hipHostMallocDefault) != hipSuccess)
@@ -758,7 +780,7 @@ int isph_destroy(isph_ctx* c) {
                  c->rq64, c->rv64, c->rq32, c->rv32, c->rsum,
                  c->nbr, c->cnt, c->val64, c->val32, c->dpos, c->rhs, c->part, c->counters,
                  c->sc, c->ss, c->err, c->spos, c->smass, c->srho, c->sstart,
-                 c->pos_nb, c->cdiff, c->salt_coef};
+                 c->pos_nb, c->cdiff, c->salt_coef, c->fsol};
     for (auto q : p)
         if (q) (void)hipFree(q);
     for (int k = 0; k < 12; ++k)
@@ -770,10 +792,12 @@ int isph_destroy(isph_ctx* c) {
     adj_free(c);
     free_zones(c);
     mg_free(c);
+    gm_free(c);
     for (auto& sl : c->slots) {
         if (!sl.v.pos) continue;
         free_state(sl.v);
         if (sl.key) (void)hipFree(sl.key);
+        if (sl.fsol) (void)hipFree(sl.fsol);
     }
     for (int k = 0; k < PROF_SLOTS; ++k)
         for (auto& e : c->prof.pool[k]) {
@@ -856,6 +880,11 @@ int isph_set_particles(isph_ctx* c, int32_t n, const double* x, const double* y,
     for (int i = 0; i < n; ++i) c->next_pid = std::max(c->next_pid, pid[i] + 1);
     int rc = sort_particles(c, false, true);
     if (rc) return rc;
+    // fsol = fguess: the uploaded pressures in the first row order
+    // (memory.f90:113-144)
+    if (n > 0)
+        HIPCHK(c, hipMemcpyAsync(c->fsol, c->st[c->cur].p, sizeof(double) * n,
+                                 hipMemcpyDeviceToDevice, c->stream));
     return sync_check(c);
 }
 
@@ -1166,10 +1195,13 @@ int isph_checkpoint(isph_ctx* c, int32_t slot) {
     if (!sl.v.pos) {
         if (alloc_state(c, sl.v, (size_t)c->cap)) return fail(c, ISPH_E_HIP, "checkpoint allocation failed");
         HIPCHK(c, hipMalloc((void**)&sl.key, sizeof(int) * c->cap));
+        HIPCHK(c, hipMalloc((void**)&sl.fsol, sizeof(double) * c->cap));
     }
     int rc = copy_state(c, c->st[c->cur], sl.v, c->n);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(sl.key, c->key_s, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sl.fsol, c->fsol, sizeof(double) * c->cap, hipMemcpyDeviceToDevice,
+                             c->stream));
     HIPCHK(c, hipMemcpyAsync(&sl.ss, c->ss, sizeof(SimScalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     sl.n = c->n;
@@ -1184,6 +1216,8 @@ int isph_restore(isph_ctx* c, int32_t slot) {
     int rc = copy_state(c, sl.v, c->st[c->cur], sl.n);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(c->key_s, sl.key, sizeof(int) * sl.n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->fsol, sl.fsol, sizeof(double) * c->cap, hipMemcpyDeviceToDevice,
+                             c->stream));
     HIPCHK(c, hipMemcpyAsync(c->ss, &sl.ss, sizeof(SimScalars), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->n = sl.n;

@@ -36,6 +36,7 @@ struct Profiler {
 
 struct AdjointState;  // isph_adjoint.hip
 struct MGData;        // isph_mg.hip
+struct GmData;        // isph_gmres.hip
 
 // a buffer zone as installed by isph_set_buffer_zones (isph_buffers.hip)
 struct BufferZone {
@@ -96,11 +97,16 @@ struct isph_ctx {
         isph::StateView v{};
         int* key = nullptr;
         isph::SimScalars ss{};
+        double* fsol = nullptr;
         int n = 0;
         bool used = false;
     } slots[64];
     isph::host::AdjointState* adj = nullptr;
     isph::host::MGData* mg = nullptr;
+    isph::host::GmData* gm = nullptr;
+    // the GMRES solve's warm start and result in row order, kept across
+    // steps and re-binnings (fsol of initialize.f90:50, memory.f90:109-144)
+    double* fsol = nullptr;
     isph::host::BufferZone zones[ISPH_MAX_ZONES];
     int nzones = 0;
     double zone_shift = 0.0;
@@ -171,6 +177,12 @@ ResidArgs resid_args(isph_ctx* c, double* x, double* r, const double* val = null
                      const double* rhs = nullptr);
 int solve_mg(isph_ctx* c);
 void mg_free(isph_ctx* c);
+
+// restarted GMRES, the reference's live solve (isph_gmres.hip); converged is
+// false when the iteration limit was reached first
+int solve_gmres(isph_ctx* c, double* x, bool& converged);
+bool prec_mixed(int precision);  // the modes with FP32 matrix values
+void gm_free(isph_ctx* c);
 
 // discrete adjoint (isph_adjoint.hip)
 void adj_free(isph_ctx* c);

@@ -221,6 +221,24 @@ __device__ __forceinline__ double block_sum_fin(double v) {
     return ((a[0] + a[1]) + (a[2] + a[3]));
 }
 
+// k_finish's reduction of one column of nbp block partials in a block of
+// FIN_BLOCK threads: thread t sums partials t, t + FIN_BLOCK, ... in order,
+// then block_sum_fin; valid in every thread
+__device__ __forceinline__ double fin_sum(const double* __restrict__ part, int nbp) {
+    double p[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int k = threadIdx.x + u * FIN_BLOCK;
+        p[u] = k < nbp ? part[k] : 0.0;
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (threadIdx.x + u * FIN_BLOCK < nbp) s = s + p[u];
+    for (int k = threadIdx.x + 8 * FIN_BLOCK; k < nbp; k += FIN_BLOCK) s = s + part[k];
+    return block_sum_fin(s);
+}
+
 // Operands of an FP64 residual b - A64 x.
 struct ResidArgs {
     int n;

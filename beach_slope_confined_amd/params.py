@@ -16,6 +16,7 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_MIXED_V64 = 2
 PREC_MIXED_MG = 3
+PREC_FGMRES = 4
 VISC_REFERENCE = 0
 VISC_CONSTANT = 1
 ORDER_CELL = 0

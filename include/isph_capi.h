@@ -64,6 +64,15 @@ extern "C" {
 #define ISPH_PREC_MIXED_MG  3  /* as MIXED_V64, right-preconditioned by one FP32
                                   V-cycle of an aggregation multigrid on the cell
                                   grid (cells, then 2x2 blocks of cells)             */
+#define ISPH_PREC_FGMRES    4  /* FP64 restarted GMRES, the reference's live solve
+                                  (solver.f90:306-347, MKL dfgmres without a
+                                  preconditioner): relative tolerance `tol` on the
+                                  residual estimate against the initial residual
+                                  (1e-3 there), zero-norm test 1e-12, restart length
+                                  inner_maxit (0: min(150, n)), at most `maxit`
+                                  iterations (150 there); warm start from the last
+                                  solution in row order (fsol); without convergence
+                                  the pressure keeps that warm start               */
 
 /* Particle order kept by cellshift/cellshiftalt (isph_params.cell_order).
  * ISPH_ORDER_CELL: by cell (column-major, y ascending), walls first, then
@@ -128,7 +137,7 @@ typedef struct isph_params {
     int32_t drag_law;    /* ISPH_DRAG_*                                         */
     int32_t precision;   /* ISPH_PREC_*                                         */
     int32_t maxit;       /* BiCGSTAB iteration cap                              */
-    int32_t inner_maxit; /* unused, 0                                           */
+    int32_t inner_maxit; /* FGMRES restart length (0: min(150, n))              */
     int32_t max_outer;   /* mixed precision: cap on reliable updates            */
     int32_t max_nbr;     /* neighbour-list capacity per particle                */
     int32_t shifting;    /* 1: opt2_shift moves particles                       */

@@ -76,6 +76,7 @@ struct ora_ctx {
     int32_t *rowptr, *col;
     double *val, *rhs;
     int assembled;
+    double *fsol;           /* GMRES warm start and result in row order */
     double *w[10];
     float *f[8];
     double umax, numax, dt, t, relres, dtsol;
@@ -303,6 +304,8 @@ static int check_params(ora_ctx* c, const isph_params* prm) {
         return set_err(c, ISPH_E_ARG, "unknown cell_order");
     if (prm->cell_order == ISPH_ORDER_REFERENCE && !(prm->cell_size >= 4.0 * prm->h))
         return set_err(c, ISPH_E_ARG, "the reference order needs cell_size >= 4h");
+    if (prm->precision < ISPH_PREC_FP64 || prm->precision > ISPH_PREC_FGMRES)
+        return set_err(c, ISPH_E_ARG, "unknown precision");
     int32_t nx, ny;
     grid_dims(prm, &nx, &ny);
     if ((double)nx * (double)ny * 2.0 > 2.0e9) return set_err(c, ISPH_E_ARG, "cell grid too large");
@@ -323,7 +326,7 @@ int ora_create(const isph_params* prm, int32_t cap, int32_t cap_soil, ora_ctx** 
     double** dp[] = {&c->x, &c->y, &c->vx, &c->vy, &c->xs, &c->ys, &c->vxs, &c->vys,
                      &c->mass, &c->rho, &c->oden, &c->p, &c->por, &c->R, &c->nut,
                      &c->tke, &c->varts, &c->vel, &c->divr, &c->tmpd, &c->rhs,
-                     &c->con, &c->ovol, &c->cdiff, &c->xnb, &c->ynb};
+                     &c->con, &c->ovol, &c->cdiff, &c->xnb, &c->ynb, &c->fsol};
     for (size_t k = 0; k < sizeof(dp) / sizeof(dp[0]); ++k) *dp[k] = (double*)xmalloc(nd);
     c->L = (double*)xmalloc(4 * nd);
     for (int k = 0; k < 10; ++k) c->w[k] = (double*)xmalloc(nd);
@@ -371,7 +374,7 @@ int ora_destroy(ora_ctx* c) {
     double* dp[] = {c->x, c->y, c->vx, c->vy, c->xs, c->ys, c->vxs, c->vys, c->mass,
                     c->rho, c->oden, c->p, c->por, c->R, c->nut, c->tke, c->varts,
                     c->vel, c->divr, c->tmpd, c->rhs, c->L, c->val, c->sx, c->sy,
-                    c->sm, c->sr, c->con, c->ovol, c->cdiff, c->xnb, c->ynb};
+                    c->sm, c->sr, c->con, c->ovol, c->cdiff, c->xnb, c->ynb, c->fsol};
     for (size_t k = 0; k < sizeof(dp) / sizeof(dp[0]); ++k) free(dp[k]);
     for (int k = 0; k < 10; ++k) free(c->w[k]);
     for (int k = 0; k < 8; ++k) free(c->f[k]);
@@ -557,7 +560,11 @@ int ora_set_particles(ora_ctx* c, int32_t n, const double* x, const double* y,
     c->next_pid = 0;
     for (int32_t i = 0; i < n; ++i)
         if (pid[i] + 1 > c->next_pid) c->next_pid = pid[i] + 1;
-    return sort_particles(c, 0, 1);
+    int rc = sort_particles(c, 0, 1);
+    if (rc) return rc;
+    /* fsol = fguess: the pressures in the first row order (memory.f90:113-144) */
+    memcpy(c->fsol, c->p, sizeof(double) * (size_t)n);
+    return ISPH_OK;
 }
 
 int ora_get_particles(ora_ctx* c, int32_t* n, double* x, double* y, double* vx,
@@ -1385,6 +1392,130 @@ static int mixed_solve(ora_ctx* c, double* x, int32_t* iters, int32_t* outer, do
     return rc;
 }
 
+/* ---------------------------------------------------------------------- */
+/* solver.f90:306-347 fgmres: MKL's RCI dfgmres as the reference sets it   */
+/* ---------------------------------------------------------------------- */
+/* No preconditioner (ipar(11) = 0), so FGMRES (Saad 1993) is restarted
+ * GMRES with z_j = v_j.  Stop tests after every iteration, in this order:
+ * the residual estimate |g_(j+1)| <= tol * ||r0|| (ipar(9) = 1, dpar(1) =
+ * tol, dpar(3) = ||r0|| of the warm start), the zero norm of the new Arnoldi
+ * vector <= 1e-12 (ipar(12) = 1, dpar(8)), the iteration limit `maxit`
+ * (ipar(5)), the end of the restart cycle of m = min(restart or 150, n)
+ * columns (ipar(15)).  The Arnoldi vectors are orthogonalised by classical
+ * Gram-Schmidt applied twice; every dot product is ora_dot_tree, so the GPU
+ * kernels' trees give the same bits.  On convergence (or a zero norm) and at
+ * the end of a cycle x += V y with y = R^-1 g; at the iteration limit x is
+ * left as it was at the start of the cycle.  Returns 1 when converged, 0 at
+ * the limit, -1 on a breakdown. */
+#define GM_DEFAULT_M 150
+#define GM_ZERO_NORM 1e-12
+static int fgmres_core(int32_t n, const int32_t* rp, const int32_t* col, const double* val,
+                       const double* b, double* x, double tol, int32_t maxit, int32_t restart,
+                       int32_t* iters, double* relres) {
+    int32_t m = restart > 0 ? restart : GM_DEFAULT_M;
+    if (m > n) m = n;
+    *iters = 0;
+    *relres = 0.0;
+    if (n == 0) return 1;
+    double* V = (double*)xmalloc(sizeof(double) * (size_t)(m + 1) * (size_t)n);
+    double* w = (double*)xmalloc(sizeof(double) * (size_t)n);
+    double* u = (double*)xmalloc(sizeof(double) * (size_t)n);
+    double* R = (double*)xmalloc(sizeof(double) * (size_t)(m + 1) * (size_t)m);
+    double* cs = (double*)xmalloc(sizeof(double) * (size_t)m);
+    double* sn = (double*)xmalloc(sizeof(double) * (size_t)m);
+    double* g = (double*)xmalloc(sizeof(double) * (size_t)(m + 1));
+    double* y = (double*)xmalloc(sizeof(double) * (size_t)m);
+    double* h1 = (double*)xmalloc(sizeof(double) * (size_t)(m + 1));
+    double* h2 = (double*)xmalloc(sizeof(double) * (size_t)(m + 1));
+    enum { RUN, CONV, HAPPY, CYCLE, MAXIT, FAIL } st;
+    /* warm-start residual */
+    spmv_csr(n, rp, col, val, x, w);
+    for (int32_t r = 0; r < n; ++r) u[r] = b[r] - w[r];
+    double rn = sqrt(ora_dot_tree(u, u, n));
+    const double r0 = rn, tolabs = tol * rn;
+    double res = rn;
+    int32_t total = 0;
+    st = !isfinite(rn) ? FAIL : (rn == 0.0 || rn <= tolabs) ? CONV : RUN;
+    while (st == RUN) {
+        double beta = rn;
+        g[0] = rn;
+        int32_t j = 0;
+        while (st == RUN) {
+            double* vj = V + (size_t)j * n;
+            for (int32_t r = 0; r < n; ++r) vj[r] = u[r] / beta;
+            spmv_csr(n, rp, col, val, vj, w);
+            for (int32_t i = 0; i <= j; ++i) h1[i] = ora_dot_tree(V + (size_t)i * n, w, n);
+            for (int32_t r = 0; r < n; ++r) {
+                double t = w[r];
+                for (int32_t i = 0; i <= j; ++i) t = t - h1[i] * V[(size_t)i * n + r];
+                u[r] = t;
+            }
+            for (int32_t i = 0; i <= j; ++i) h2[i] = ora_dot_tree(V + (size_t)i * n, u, n);
+            for (int32_t r = 0; r < n; ++r) {
+                double t = u[r];
+                for (int32_t i = 0; i <= j; ++i) t = t - h2[i] * V[(size_t)i * n + r];
+                u[r] = t;
+            }
+            beta = sqrt(ora_dot_tree(u, u, n));
+            double* cj = R + (size_t)j * (m + 1);
+            for (int32_t i = 0; i <= j; ++i) cj[i] = h1[i] + h2[i];
+            for (int32_t i = 0; i < j; ++i) {
+                const double a = cj[i], bb = cj[i + 1];
+                cj[i] = cs[i] * a + sn[i] * bb;
+                cj[i + 1] = cs[i] * bb - sn[i] * a;
+            }
+            const double a = cj[j];
+            const double d = sqrt(a * a + beta * beta);
+            total++;
+            if (!(d > 0.0) || !isfinite(d)) { st = FAIL; break; }
+            cs[j] = a / d;
+            sn[j] = beta / d;
+            cj[j] = d;
+            const double gj = g[j];
+            g[j + 1] = -(sn[j] * gj);
+            g[j] = cs[j] * gj;
+            j++;
+            res = fabs(g[j]);
+            if (!isfinite(res)) st = FAIL;
+            else if (res <= tolabs) st = CONV;
+            else if (beta <= GM_ZERO_NORM) st = HAPPY;
+            else if (total >= maxit) st = MAXIT;
+            else if (j >= m) st = CYCLE;
+        }
+        if (st == CONV || st == HAPPY || st == CYCLE) {
+            for (int32_t i = j - 1; i >= 0; --i) {
+                double sacc = g[i];
+                for (int32_t l = i + 1; l < j; ++l) sacc = sacc - R[(size_t)l * (m + 1) + i] * y[l];
+                y[i] = sacc / R[(size_t)i * (m + 1) + i];
+            }
+            for (int32_t r = 0; r < n; ++r) {
+                double t = x[r];
+                for (int32_t i = 0; i < j; ++i) t = t + y[i] * V[(size_t)i * n + r];
+                x[r] = t;
+            }
+        }
+        if (st == CYCLE) {
+            spmv_csr(n, rp, col, val, x, w);
+            for (int32_t r = 0; r < n; ++r) u[r] = b[r] - w[r];
+            rn = sqrt(ora_dot_tree(u, u, n));
+            res = rn;
+            st = !isfinite(rn) ? FAIL : (rn == 0.0 || rn <= tolabs) ? CONV : RUN;
+        }
+    }
+    *iters = total;
+    *relres = r0 > 0.0 ? res / r0 : 0.0;
+    free(V); free(w); free(u); free(R); free(cs); free(sn); free(g); free(y); free(h1); free(h2);
+    return st == FAIL ? -1 : (st == MAXIT ? 0 : 1);
+}
+
+int ora_fgmres_csr(int32_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+                   const double* b, double* x, double tol, int32_t maxit, int32_t restart,
+                   int32_t* iters, double* relres, int32_t* converged) {
+    int rc = fgmres_core(n, rowptr, col, val, b, x, tol, maxit, restart, iters, relres);
+    if (converged) *converged = rc == 1;
+    return rc < 0 ? ISPH_E_NOCONV : ISPH_OK;
+}
+
 /* solve (isph.f90:460) and pressure scatter (isph.f90:465-483); the warm
  * start is the particles' current pressure (the reference warm-starts
  * FGMRES from fsol).  A failed solve leaves p unchanged, as the reference's
@@ -1400,6 +1531,23 @@ int ora_ppe_solve(ora_ctx* c, int32_t* iters, double* relres) {
     if (prm->precision == ISPH_PREC_MIXED_MG)
         return set_err(c, ISPH_E_ARG, "the multigrid solve is checked against the FP64 solve, "
                                       "it has no restatement here");
+    if (prm->precision == ISPH_PREC_FGMRES) {
+        /* warm start fsol in row order (solver.f90:319); fsol takes the
+         * solution only on convergence (solver.f90:330-333); p = fsol */
+        memcpy(x, c->fsol, sizeof(double) * (size_t)c->n);
+        int32_t conv = 0;
+        rc = ora_fgmres_csr(c->n, c->rowptr, c->col, c->val, c->rhs, x, prm->tol, prm->maxit,
+                            prm->inner_maxit, &it, &rel, &conv);
+        c->iters = it;
+        c->outer = 0;
+        c->relres = rel;
+        if (iters) *iters = it;
+        if (relres) *relres = rel;
+        if (rc != ISPH_OK) return set_err(c, rc, "GMRES breakdown (non-finite or singular)");
+        if (conv) memcpy(c->fsol, x, sizeof(double) * (size_t)c->n);
+        memcpy(c->p, c->fsol, sizeof(double) * (size_t)c->n);
+        return ISPH_OK;
+    }
     if (prm->precision == ISPH_PREC_MIXED || prm->precision == ISPH_PREC_MIXED_V64) {
         rc = mixed_solve(c, x, &it, &outer, &rel);
     } else {

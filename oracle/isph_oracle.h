@@ -93,6 +93,13 @@ double ora_dot_tree(const double* a, const double* b, int64_t n);
 int ora_bicgstab_csr(int32_t n, const int32_t* rowptr, const int32_t* col,
                      const double* val, const double* b, double* x, double tol,
                      int32_t maxit, int32_t* iters, double* relres);
+/* restarted GMRES of the reference's live solve (solver.f90:306-347):
+ * restart 0 = min(150, n); x is the warm start and, when converged (or at
+ * the end of a restart cycle), the solution; converged = 0 at maxit */
+int ora_fgmres_csr(int32_t n, const int32_t* rowptr, const int32_t* col,
+                   const double* val, const double* b, double* x, double tol,
+                   int32_t maxit, int32_t restart, int32_t* iters, double* relres,
+                   int32_t* converged);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,9 @@ def load():
         lib.ora_bicgstab_csr.argtypes = [ctypes.c_int32, ip, ip, dp, dp, dp, ctypes.c_double,
                                          ctypes.c_int32, ip, dp]
         lib.ora_bicgstab_csr.restype = ctypes.c_int
+        lib.ora_fgmres_csr.argtypes = [ctypes.c_int32, ip, ip, dp, dp, dp, ctypes.c_double,
+                                       ctypes.c_int32, ctypes.c_int32, ip, dp, ip]
+        lib.ora_fgmres_csr.restype = ctypes.c_int
         _LIB = lib
     return _LIB
 
@@ -97,3 +100,26 @@ def bicgstab_csr(rowptr, col, val, b, x0, tol=1e-10, maxit=1000):
                                  x.ctypes.data_as(dp), tol, maxit, ctypes.byref(it),
                                  ctypes.byref(rel))
     return rc, x, it.value, rel.value
+
+
+def fgmres_csr(rowptr, col, val, b, x0, tol, maxit, restart=0):
+    """The oracle's GMRES of the reference's live solve on a CSR matrix:
+    returns (rc, x, iterations, residual estimate / ||r0||, converged).  x is
+    the warm start x0 where the iteration limit came first within the first
+    restart cycle."""
+    import numpy as np
+    ip = ctypes.POINTER(ctypes.c_int32)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.ascontiguousarray(x0, np.float64).copy()
+    it = ctypes.c_int32()
+    rel = ctypes.c_double()
+    conv = ctypes.c_int32()
+    rc = load().ora_fgmres_csr(len(b), rowptr.ctypes.data_as(ip), col.ctypes.data_as(ip),
+                               val.ctypes.data_as(dp), b.ctypes.data_as(dp), x.ctypes.data_as(dp),
+                               tol, maxit, restart, ctypes.byref(it), ctypes.byref(rel),
+                               ctypes.byref(conv))
+    return rc, x, it.value, rel.value, bool(conv.value)
