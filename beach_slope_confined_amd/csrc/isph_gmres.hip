@@ -40,7 +40,8 @@ namespace isph {
 constexpr int GM_DEFAULT_M = 150;       // MKL's default restart length and limit
 constexpr int GM_MAXM = 512;            // longest restart cycle accepted
 constexpr double GM_ZERO_NORM = 1e-12;  // dpar(8)
-constexpr int GM_CHUNK = 4;             // basis vectors per block reduction round
+constexpr int GM_CHUNK = 8;             // basis vectors per block reduction round
+constexpr int GM_SUB = 8;               // basis vectors loaded ahead in a subtraction
 constexpr int GM_BATCH = 8;             // Arnoldi steps enqueued between host reads
 
 enum { GM_RUN = 0, GM_CONV = 1, GM_HAPPY = 2, GM_CYCLE = 3, GM_MAXIT = 4, GM_FAIL = 5 };
@@ -98,6 +99,21 @@ __device__ __forceinline__ void gm_dots(bool valid, int r, int j, const double* 
             for (int k = 0; k < GM_CHUNK; ++k)
                 if (i0 + k <= j) part[(size_t)(i0 + k) * nbp + lb] = v[k];
     }
+}
+
+// x - h_0 v_0 - h_1 v_1 - ... - h_j v_j at row r, subtracted in order; the
+// loads of GM_SUB basis vectors are issued ahead of their subtractions
+__device__ __forceinline__ double gm_subtract(double x, int r, int j, const double* __restrict__ V,
+                                              size_t cap, const double* __restrict__ h) {
+    for (int i0 = 0; i0 <= j; i0 += GM_SUB) {
+        double v[GM_SUB];
+#pragma unroll
+        for (int k = 0; k < GM_SUB; ++k) v[k] = i0 + k <= j ? V[(size_t)(i0 + k) * cap + r] : 0.0;
+#pragma unroll
+        for (int k = 0; k < GM_SUB; ++k)
+            if (i0 + k <= j) x = x - h[i0 + k] * v[k];
+    }
+    return x;
 }
 
 // u = b - A x, partial (u, u)
@@ -188,8 +204,7 @@ __global__ void __launch_bounds__(BLOCK) k_gm_orth(int n, int j, const double* _
     const bool valid = r < n;
     double x = 0.0;
     if (valid) {
-        x = w[r];
-        for (int i = 0; i <= j; ++i) x = x - h[i] * V[(size_t)i * cap + r];
+        x = gm_subtract(w[r], r, j, V, cap, h);
         u[r] = x;
     }
     gm_dots(valid, r, j, V, cap, x, part, gridDim.x, lb);
@@ -205,8 +220,7 @@ __global__ void __launch_bounds__(BLOCK) k_gm_orth2(int n, int j, const double* 
     const int r = lb * BLOCK + threadIdx.x;
     double q = 0.0;
     if (r < n) {
-        double x = u[r];
-        for (int i = 0; i <= j; ++i) x = x - h[i] * V[(size_t)i * cap + r];
+        const double x = gm_subtract(u[r], r, j, V, cap, h);
         u[r] = x;
         q = x * x;
     }
@@ -226,17 +240,29 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_gm_givens(int j, int m, int maxit
                                                          double* __restrict__ sn,
                                                          double* __restrict__ g, GmState* st) {
     if (st->done) return;
-    const double t = fin_sum(part, nbp);
+    // the column and the earlier rotations staged in LDS for thread 0
+    __shared__ double scol[GM_MAXM + 1], scs[GM_MAXM], ssn[GM_MAXM];
+    for (int i = threadIdx.x; i <= j; i += FIN_BLOCK) {
+        scol[i] = h1[i] + h2[i];
+        if (i < j) {
+            scs[i] = cs[i];
+            ssn[i] = sn[i];
+        }
+    }
+    const double t = fin_sum(part, nbp);  // its barriers also order the staging
+    double* col = R + (size_t)j * (m + 1);
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < j; ++i) {
+            const double a = scol[i], b = scol[i + 1];
+            scol[i] = scs[i] * a + ssn[i] * b;
+            scol[i + 1] = scs[i] * b - ssn[i] * a;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < j; i += FIN_BLOCK) col[i] = scol[i];
     if (threadIdx.x != 0) return;
     const double beta = sqrt(t);
-    double* col = R + (size_t)j * (m + 1);
-    for (int i = 0; i <= j; ++i) col[i] = h1[i] + h2[i];
-    for (int i = 0; i < j; ++i) {
-        const double a = col[i], b = col[i + 1];
-        col[i] = cs[i] * a + sn[i] * b;
-        col[i + 1] = cs[i] * b - sn[i] * a;
-    }
-    const double a = col[j];
+    const double a = scol[j];
     const double d = sqrt(a * a + beta * beta);
     st->beta = beta;
     st->total = st->total + 1;

@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-reference-run", action="store_true")
     ap.add_argument("--mg-only", action="store_true",
                     help="time only the multigrid-preconditioned step (for profiling)")
+    ap.add_argument("--fgmres-only", action="store_true",
+                    help="time only the step with the reference's FGMRES solve (for profiling)")
     ap.add_argument("--reference-run-only", action="store_true")
     ap.add_argument("--adjoint-dx", type=float, default=0.002)
     ap.add_argument("--adjoint-steps", type=int, default=50)
@@ -77,12 +79,12 @@ def kernel_bytes(kind, nnz_off, n, val_bytes, vec_bytes, idx_bytes=4):
     return idx + val + vec
 
 
-def run_gpu(cfg, steps, warmup, precision, profile, tol=None):
+def run_gpu(cfg, steps, warmup, precision, profile, tol=None, maxit=200000):
     from beach_slope_confined_amd import capi, configs
     from beach_slope_confined_amd.params import PREC_FP64
     c = dict(cfg)
     c["params"] = cfg["params"].copy(precision=precision, tol=tol or cfg["params"].tol,
-                                     maxit=200000)
+                                     maxit=maxit)
     sim = configs.make_sim(c, capi.IsphSim)
     # every step starts from the same device-resident input (the workload is
     # one ISPH step at scale; the snapshot copy is device-to-device)
@@ -360,7 +362,8 @@ def main():
     import torch
     torch.cuda.init()
     from beach_slope_confined_amd import configs
-    from beach_slope_confined_amd.params import PREC_FP64, PREC_MIXED, PREC_MIXED_MG, PREC_MIXED_V64
+    from beach_slope_confined_amd.params import (PREC_FGMRES, PREC_FP64, PREC_MIXED, PREC_MIXED_MG,
+                                                 PREC_MIXED_V64)
 
     cfg = configs.tank_1m(args.nside)
     if args.pmc_run:
@@ -386,6 +389,15 @@ def main():
         sal = run_salinity(args.salt_dx, args.steps, args.warmup)
         sal["k_scalart_tank"] = run_scalart_at_scale(cfg)
         print(json.dumps({"salinity": sal}), flush=True)
+        return
+
+    if args.fgmres_only:
+        rgm = run_gpu(cfg, args.steps, args.warmup, PREC_FGMRES, False, tol=1e-3, maxit=150)
+        rgm["sim"].close()
+        print(json.dumps({"fgmres_reference": {
+            "value": round(n_fluid * args.steps / rgm["seconds"], 1),
+            "ms_per_step": round(1e3 * rgm["seconds"] / args.steps, 3), "tol": 1e-3,
+            "iters_per_step": rgm["iters"][args.warmup:], "relres": rgm["relres"]}}), flush=True)
         return
 
     if args.mg_only:
@@ -502,6 +514,24 @@ def main():
                 })
             except Exception as e:
                 line["mixed_mg"]["tight_error"] = str(e)
+
+    if not args.no_mixed:
+        # the reference's live solve (FGMRES, tol 1e-3 on the residual
+        # estimate, at most 150 iterations, warm start fsol) on the same input
+        try:
+            rgm = run_gpu(cfg, args.steps, args.warmup, PREC_FGMRES, False, tol=1e-3, maxit=150)
+            rgm["sim"].close()
+            vgm = n_fluid * args.steps / rgm["seconds"]
+            pe, ve = step_errors(r64["first"], rgm["first"])
+            line["fgmres_reference"] = {
+                "value": round(vgm, 1), "ms_per_step": round(1e3 * rgm["seconds"] / args.steps, 3),
+                "speedup_vs_fp64": round(vgm / value, 3), "tol": 1e-3, "maxit": 150,
+                "iters_per_step": rgm["iters"][args.warmup:], "relres": rgm["relres"],
+                "converged": bool(rgm["relres"] <= 1e-3),
+                "pressure_err_vs_fp64": pe, "velocity_err_vs_fp64": ve,
+            }
+        except Exception as e:  # reported, not hidden
+            line["fgmres_reference"] = {"error": str(e)}
 
     if not args.no_salinity:
         try:
