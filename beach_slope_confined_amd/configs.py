@@ -137,6 +137,45 @@ def dam_break_reference(dx=0.01, seed=SEED):
     return dam_break(dx=dx, seed=seed, reference_physics=True)
 
 
+def beach_large(side=2048, dx=0.0005, seed=SEED, jitter=0.05):
+    """The large beach run: a square water column of side x side particles
+    (4,194,304 at the default) at spacing dx, in a tank 4 column widths long
+    and 2 high, with the dam break's 1:3 porous slope (toe at half a column
+    width), porosity 0.4, d50 0.01, alpha 200, beta 1.1."""
+    w = side * dx
+    L, H = 4.0 * w, 2.0 * w
+    toe = 0.5 * w
+    nx, ny = int(round(L / dx)), int(round(H / dx))
+    ii, jj = np.meshgrid(np.arange(side), np.arange(side), indexing="ij")
+    ii = ii.ravel().astype(np.float64)
+    jj = jj.ravel().astype(np.float64)
+    nf = len(ii)
+    jit = jitter * dx
+    fx = (ii + 0.5) * dx + (2.0 * uniform(nf, 1, seed) - 1.0) * jit
+    fy = (jj + 0.5) * dx + (2.0 * uniform(nf, 2, seed) - 1.0) * jit
+    wx, wy, wtid = _walls(nx, ny, dx, 0.0, 0.0, top=True)
+    rho0, g = 1000.0, -9.81
+    fp = rho0 * abs(g) * np.maximum(w - fy, 0.0)
+    parts = _assemble(fx, fy, np.zeros(nf), np.zeros(nf), wx, wy, wtid, rho0, dx, fp,
+                      np.zeros(len(wx)))
+    por = 0.4
+    in_pores = np.concatenate([np.zeros(len(wx), bool), fy < (fx - toe) / 3.0])
+    parts["mass"][in_pores] = por * rho0 * dx * dx
+    si, sj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    sx = (si.ravel() + 0.5) * dx
+    sy = (sj.ravel() + 0.5) * dx
+    keep = sy < (sx - toe) / 3.0
+    sx, sy = sx[keep], sy[keep]
+    soil = dict(x=sx, y=sy, mass=np.full(len(sx), 1800.0 * dx * dx),
+                density=np.full(len(sx), 1800.0))
+    m = 3.5 * dx
+    prm = make_params(kernel=KERNEL_WENDLAND_C2, drag_law=DRAG_ERGUN, visc_law=VISC_CONSTANT,
+                      h=1.3 * dx, dl=dx, dt=1e-4, por=por, alpha=200.0, beta=1.1, d50=0.01,
+                      xmin=-m, ymin=-m, xmax=L + m, ymax=H + m)
+    return dict(name=f"beach_large_{side}x{side}_dx{dx:g}", parts=parts, soil=soil, params=prm,
+                n_fluid=nf)
+
+
 def fourier_velocity(x, y, seed=SEED, modes=16, amp=0.1, noise=1e-3):
     """vx = sum_k (amp/modes) cos(2 pi (kx x + ky y) + phi_k), vy likewise with
     sin, integer wavevectors in [1, 8]^2, plus U(-noise, noise)."""
