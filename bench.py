@@ -67,6 +67,21 @@ def parse():
     return ap.parse_args()
 
 
+def model_spmv_bytes(n, nnz_off):
+    """Algorithmic bytes of one FP64 SpMV in the per-row model of the ISPH
+    step: 32 + 12 K bytes per row (int32 column + FP64 value per entry; row
+    pointer, x and y), K = off-diagonal entries per row."""
+    return 32 * n + 12 * nnz_off
+
+
+def model_step_bytes(n, nnz_off, iters):
+    """Algorithmic bytes of one FP64 step in the same model: per row the
+    fixed stages 1284 + 60 K and per BiCGSTAB iteration 248 + 24 K (two
+    SpMVs, five dots, a norm, three axpys)."""
+    k = nnz_off / n
+    return n * ((1284 + 60 * k) + iters * (248 + 24 * k))
+
+
 def kernel_bytes(kind, nnz_off, n, val_bytes, vec_bytes, idx_bytes=4):
     """Algorithmic bytes of one launch of a BiCGSTAB SpMV kernel: every
     neighbour index (4-byte column or 2-byte offset) and matrix value once,
@@ -423,7 +438,10 @@ def main():
     dom = max(kt, key=lambda k: kt[k][1])
     launches, tot = kt[dom]
     avg_s = tot / max(launches, 1) / 1e3
-    nbytes = kernel_bytes(dom, r64["nnz_off"], r64["n"], 8, 8, r64["idx_bytes"])
+    # algorithmic bytes: the per-row model (32 + 12 K per SpMV row); the
+    # kernel's own count (16-bit offsets, vectors once) is reported beside it
+    nbytes = model_spmv_bytes(r64["n"], r64["nnz_off"])
+    own = kernel_bytes(dom, r64["nnz_off"], r64["n"], 8, 8, r64["idx_bytes"])
     achieved = nbytes / avg_s / 1e9 if launches else 0.0
     ti = "short" if r64["idx_bytes"] == 2 else "int"
     kname = (f"k_krylov_1t<double, double, {ti}, false>" if dom == "p_spmv"
@@ -431,7 +449,15 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": kname, "launches": launches, "avg_us": round(avg_s * 1e6, 2),
-                "bytes_per_launch": nbytes}
+                "bytes_per_launch": nbytes,
+                "bytes_model": "32 + 12K per row, K = %.3f off-diagonal entries per row"
+                               % (r64["nnz_off"] / r64["n"]),
+                "own_bytes_per_launch": own,
+                "own_frac": round(own / avg_s / 1e9 / HBM_PEAK_GBS, 4) if launches else None}
+    sb = model_step_bytes(r64["n"], r64["nnz_off"], timed_iters[-1])
+    step_model = {"bytes_per_step": round(sb), "achieved": round(sb / (ms / 1e3) / 1e9, 1),
+                  "unit": "GB/s", "frac": round(sb / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "model": "rows x (1284 + 60K + iterations x (248 + 24K)) bytes"}
     pmc = os.path.join(ROOT, "profiles", "bicg_spmv_traffic.json")
     if os.path.exists(pmc):
         with open(pmc) as f:
@@ -450,7 +476,7 @@ def main():
                    "kernel": "wendland_c2", "h_over_dx": 1.3, "tol": 1e-8},
         "ppe_iters_per_step": timed_iters, "relres": r64["relres"],
         "particle_steps_per_s_total": round(n_total * args.steps / r64["seconds"], 1),
-        "roofline": roofline,
+        "roofline": roofline, "step_model": step_model,
     }
 
     if not args.no_mixed:
