@@ -192,9 +192,34 @@ def run_salinity(dx, steps, warmup):
     avg_s = tot / max(launches, 1) / 1e3
     gbs = nb / avg_s / 1e9 if launches else 0.0
     con = sim.get_field("con")[tid == 3]
+    pid = sim.get_particles()["pid"][tid == 3]
     sim.close()
     n_fluid = cfg["n_fluid"]
     n = len(cfg["parts"]["x"])
+    # the same steps with the multigrid-preconditioned PPE (to 1e-9), no
+    # profiling events; con compared particle by particle with the FP64 run
+    from beach_slope_confined_amd.params import PREC_MIXED_MG
+    cmg = configs.dam_break(dx)
+    cmg["params"] = cmg["params"].copy(salinity=1, precision=PREC_MIXED_MG, tol=1e-9)
+    sim = configs.make_sim(cmg, capi.IsphSim)
+    sim.salt_init(0.2, 0.1, 0.3)
+    for _ in range(max(warmup, 1)):
+        sim.step(1)
+    torch.cuda.synchronize()
+    m0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step(1)
+    torch.cuda.synchronize()
+    m1 = time.perf_counter()
+    pm = sim.get_particles()
+    fm = pm["tid"] == 3
+    con_mg = dict(zip(pm["pid"][fm].tolist(), sim.get_field("con")[fm].tolist()))
+    dcon = max(abs(con_mg[k] - v) for k, v in zip(pid.tolist(), con.tolist()))
+    mg = {"ms_per_step": round(1e3 * (m1 - m0) / steps, 3),
+          "value": round(n_fluid * steps / (m1 - m0), 1),
+          "ppe_iters_last_step": sim.scalars()["iters"], "tol": 1e-9,
+          "max_con_diff_vs_fp64": dcon}
+    sim.close()
     return {"workload": f"dam_break dx={dx:g}: {n_fluid} fluid, {n} particles, "
                         f"{len(cfg['soil']['x'])} soil; salt-water step, salt above "
                         f"y - 0.1 = 0.3 (x - 0.2)",
@@ -204,7 +229,8 @@ def run_salinity(dx, steps, warmup):
             "con_range": [float(con.min()), float(con.max())],
             "k_scalart": {"launches": launches, "avg_us": round(avg_s * 1e6, 2),
                           "bytes_per_launch": nb, "achieved_GBs": round(gbs, 1),
-                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}}
+                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)},
+            "multigrid": mg}
 
 
 def run_scalart_at_scale(cfg, reps=20):
