@@ -13,8 +13,9 @@
 //   comp_pos -> comp_vel -> PPE (transposed solve, assembly) -> v* ->
 //   varts -> eddyvis -> effpor -> KGC -> re-binning -> projection
 // and at the end the initial porous closure.  The PPE adjoint is
-// lambda = A^-T pbar, solved by the forward's BiCGSTAB on the transposed
-// values (the lists are symmetric, so A^T has A's sparsity).  Parameter
+// lambda = A^-T pbar, solved by the forward's BiCGSTAB (or its multigrid-
+// preconditioned form) on the transposed values (the lists are symmetric,
+// so A^T has A's sparsity).  Parameter
 // gradients accumulate per particle slot and are summed in a fixed order.
 #include <hip/hip_runtime.h>
 
@@ -68,6 +69,8 @@ struct AdjointState {
     double* w = nullptr;
     double* lam = nullptr;
     double* valT = nullptr;
+    float* val32T = nullptr;  // FP32 values and FP64 row sums of A^T for the
+    double* rsumT = nullptr;  // multigrid-preconditioned transposed solve
     double* dsc = nullptr;   // [0] aumax, [1] J, [2..25] objective sums, [26..29] gradients
     double* part = nullptr;  // 3 * OBJ_MAXP * nblk(cap)
 };
@@ -382,7 +385,35 @@ int adj_ppe(isph_ctx* c, const AdjView& a, const StateView& s, int* nbr, int* cn
     k_transpose_vals<<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, nbr, cnt, c->val64, S->valT, c->err);
     HIPCHK(c, hipMemsetAsync(S->lam, 0, sizeof(double) * n, c->stream));
     int rc = take_errors(c);
-    if (!rc) rc = solve_fp64_on(c, S->valT, a.p, S->lam);
+    if (!rc && pr.precision == ISPH_PREC_MIXED_MG) {
+        // the forward solves with the multigrid preconditioner; so does the
+        // transposed solve, on A^T's own Galerkin hierarchy
+        if (!S->val32T) {
+            const size_t N = (size_t)c->cap, M = (size_t)pr.max_nbr;
+            if (dalloc(c, &S->val32T, N * M) || dalloc(c, &S->rsumT, N))
+                rc = fail(c, ISPH_E_HIP, "adjoint buffer allocation failed");
+        }
+        if (!rc) {
+            // the hierarchy reads the bin of every row: the taped state is
+            // sorted by the keys of its positions, which have not moved
+            // since that step's re-binning, so the keys and bin starts are
+            // recomputed from them (the forward recomputation re-sorts)
+            const int64_t nb = c->grid.nbins;
+            HIPCHK(c, hipMemsetAsync(c->bincnt, 0, sizeof(int) * (nb + 2), c->stream));
+            k_bin_keys<<<nblk(n), BLOCK, 0, c->stream>>>(n, c->grid, s.pos, s.tid, s.flags, 0, 0.0,
+                                                          0.0, nullptr, c->key_s, c->bincnt,
+                                                          c->rank, c->err);
+            rc = scan(c, c->bincnt, nb + 2);
+        }
+        if (!rc) rc = take_errors(c);
+        if (!rc) {
+            k_mg_lower<<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, cnt, c->dpos, S->valT, S->val32T,
+                                                          S->rsumT);
+            rc = solve_mg_on(c, MGSystem{S->valT, S->val32T, S->rsumT, a.p, S->lam});
+        }
+    } else if (!rc) {
+        rc = solve_fp64_on(c, S->valT, a.p, S->lam);
+    }
     if (!rc) rc = read_sc(c);
     if (!rc && iters) *iters += c->sc_host->total;
     c->nbr = nbr0;
@@ -492,7 +523,7 @@ void adj_free(isph_ctx* c) {
     if (S->have_adj) {
         free_adjview(S->A[0]);
         free_adjview(S->A[1]);
-        void* p[] = {S->th, S->aum, S->w, S->lam, S->valT, S->dsc, S->part};
+        void* p[] = {S->th, S->aum, S->w, S->lam, S->valT, S->val32T, S->rsumT, S->dsc, S->part};
         for (auto q : p)
             if (q) (void)hipFree(q);
     }

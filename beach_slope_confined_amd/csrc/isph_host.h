@@ -178,6 +178,18 @@ ResidArgs resid_args(isph_ctx* c, double* x, double* r, const double* val = null
 int solve_mg(isph_ctx* c);
 void mg_free(isph_ctx* c);
 
+// a linear system for the multigrid solve: the FP64 and FP32 values and
+// FP64 row sums of a Jacobi-scaled matrix on the current lists, the
+// right-hand side and the solution (holding the initial guess)
+struct MGSystem {
+    const double* val64;
+    const float* val32;
+    const double* rsum;
+    const double* rhs;
+    double* x;
+};
+int solve_mg_on(isph_ctx* c, const MGSystem& sys);
+
 // restarted GMRES, the reference's live solve (isph_gmres.hip); converged is
 // false when the iteration limit was reached first
 int solve_gmres(isph_ctx* c, double* x, bool& converged);

@@ -317,6 +317,8 @@ template <class TX>
 __global__ void k_mx_flush(int n, double* x64, TX* xs);
 __global__ void k_to_short(int n, size_t ld, const int* nbr, const int* cnt, short* out,
                            uint32_t* err);
+__global__ void k_mg_lower(int n, size_t ld, const int* cnt, const int* dpos, const double* val,
+                           float* val32, double* rsum);
 __global__ void k_comp_vel(int n, KernelConsts kc, size_t ld, StateView s, const int* nbr,
                            const int* cnt, const double* L, const SimScalars* ss, uint32_t* err);
 __global__ void k_comp_pos(int n, StateView s, const SimScalars* ss);

@@ -569,6 +569,27 @@ __global__ void __launch_bounds__(BLOCK) k_mg_x(int n, const double* __restrict_
     }
 }
 
+// FP32 values and FP64 row sums of a Jacobi-scaled FP64 matrix in the
+// assembly's layout (unit diagonal at dpos, row sum in column order as
+// k_assemble forms it): the mixed-precision operands of a matrix that was
+// not assembled, such as the adjoint's transpose
+__global__ void k_mg_lower(int n, size_t ld, const int* __restrict__ cnt,
+                           const int* __restrict__ dpos, const double* __restrict__ val,
+                           float* __restrict__ val32, double* __restrict__ rsum) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int c = cnt[i], dp = dpos[i];
+    double rs = 0.0;
+    for (int k = 0; k < c; ++k) {
+        const double v = val[(size_t)k * ld + i];
+        val32[(size_t)k * ld + i] = (float)v;
+        if (k == dp) rs = rs + 1.0;
+        rs = rs + v;
+    }
+    if (dp == c) rs = rs + 1.0;
+    rsum[i] = rs;
+}
+
 namespace host {
 
 struct MGLevel {
@@ -659,15 +680,15 @@ static int mg_alloc(isph_ctx* c) {
     return ISPH_OK;
 }
 
-// Galerkin hierarchy of the current matrix
-static int mg_setup(isph_ctx* c) {
+// Galerkin hierarchy of the system's matrix
+static int mg_setup(isph_ctx* c, const MGSystem& sys) {
     int rc = mg_alloc(c);
     if (rc) return rc;
     MGData* m = c->mg;
     const GridInfo& g = c->grid;
     k_mg_cells<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, g, c->key_s, m->cell);
     k_mg_galerkin1<<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
-        g, c->bincnt, (size_t)c->cap, c->nbr, c->cnt, c->val64, m->cell, m->lv[0].A, c->err);
+        g, c->bincnt, (size_t)c->cap, c->nbr, c->cnt, sys.val64, m->cell, m->lv[0].A, c->err);
     for (int l = 0; l + 1 < m->nlev; ++l) {
         const MGLevel &a = m->lv[l], &b = m->lv[l + 1];
         k_mg_galerkin<<<nblk((int64_t)b.nx * b.ny), BLOCK, 0, c->stream>>>(a.nx, a.ny, a.A, b.nx,
@@ -680,13 +701,14 @@ static int mg_setup(isph_ctx* c) {
 // one V-cycle: b formed from `pairs` (MODE 0: p from rq, MODE 1: s from rv)
 // into bout, preconditioned vector into out
 template <class TI, int MODE, bool FIRST>
-static void mg_vcycle(isph_ctx* c, const TI* nbr, const double2* pairs, double* bout, double* out) {
+static void mg_vcycle(isph_ctx* c, const MGSystem& sys, const TI* nbr, const double2* pairs,
+                      double* bout, double* out) {
     MGData* m = c->mg;
     const GridInfo& g = c->grid;
     const int n = c->n;
     const size_t ld = (size_t)c->cap;
     k_mg_fine_down<TI, MODE, FIRST><<<nblk(n), BLOCK, 0, c->stream>>>(
-        n, ld, nbr, c->cnt, c->val32, pairs, c->sc, bout, m->x0, m->r0);
+        n, ld, nbr, c->cnt, sys.val32, pairs, c->sc, bout, m->x0, m->r0);
     k_mg_restrict1<<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(g, c->bincnt, m->r0,
                                                                          m->lv[0].b, c->sc);
     const int lb = m->lb;
@@ -705,12 +727,13 @@ static void mg_vcycle(isph_ctx* c, const TI* nbr, const double2* pairs, double* 
     }
     k_mg_prolong1<<<nblk(n), BLOCK, 0, c->stream>>>(n, m->cell, m->lv[0].x, MG_OVER, m->x0,
                                                     c->sc);
-    k_mg_fine_up<TI><<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, c->val32, bout, m->x0,
+    k_mg_fine_up<TI><<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, sys.val32, bout, m->x0,
                                                         out, c->sc);
 }
 
 template <class TI>
-static int mg_iterate(isph_ctx* c, const TI* nbr, const SolveCtl& ctl, bool& first) {
+static int mg_iterate(isph_ctx* c, const MGSystem& sys, const TI* nbr, const SolveCtl& ctl,
+                      bool& first) {
     const int grid = nblk8(c->n);
     const int n = c->n;
     const size_t ld = (size_t)c->cap;
@@ -728,14 +751,14 @@ static int mg_iterate(isph_ctx* c, const TI* nbr, const SolveCtl& ctl, bool& fir
         }
         for (int b = 0; b < batch; ++b) {
             if (first)
-                mg_vcycle<TI, 0, true>(c, nbr, c->rq64, p, ph);
+                mg_vcycle<TI, 0, true>(c, sys, nbr, c->rq64, p, ph);
             else
-                mg_vcycle<TI, 0, false>(c, nbr, c->rq64, p, ph);
-            k_mg_v<TI><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, c->val32, c->rsum, ph,
+                mg_vcycle<TI, 0, false>(c, sys, nbr, c->rq64, p, ph);
+            k_mg_v<TI><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, sys.val32, sys.rsum, ph,
                                                        c->rq64, rhat, c->rv64, c->part, c->sc);
             k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_ALPHA, grid, c->part, c->sc, ctl);
-            mg_vcycle<TI, 1, false>(c, nbr, c->rv64, s, sh);
-            k_mg_t<TI><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, c->val32, c->rsum, sh, s,
+            mg_vcycle<TI, 1, false>(c, sys, nbr, c->rv64, s, sh);
+            k_mg_t<TI><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, sys.val32, sys.rsum, sh, s,
                                                        t, c->part, c->sc);
             k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_OMEGA, grid, c->part, c->sc, ctl);
             k_mg_x<<<grid, BLOCK, 0, c->stream>>>(n, p, ph, s, sh, t, c->rv64, rhat, xs, c->rq64,
@@ -751,15 +774,15 @@ static int mg_iterate(isph_ctx* c, const TI* nbr, const SolveCtl& ctl, bool& fir
 }
 
 template <class TI>
-static int solve_mg_t(isph_ctx* c, const TI* nbr) {
+static int solve_mg_t(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
     const int grid = nblk8(c->n);
-    double* x = c->v64[8];
+    double* x = sys.x;
     double* r64 = c->v64[9];
-    int rc = mg_setup(c);
+    int rc = mg_setup(c, sys);
     if (rc) return rc;
     SolveCtl ctl = solve_ctl(c, 1);
     SolveCtl ctl_flush = solve_ctl(c, 2);
-    ResidArgs ra = resid_args(c, x, r64);
+    ResidArgs ra = resid_args(c, x, r64, sys.val64, sys.rhs);
     k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 0, nullptr, nullptr);
     k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, solve_ctl(c, 0));
     k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_RESID, grid, c->part, c->sc, ctl);
@@ -777,7 +800,7 @@ static int solve_mg_t(isph_ctx* c, const TI* nbr) {
     if ((rc = read_sc(c))) return rc;
     bool first = true;
     for (;;) {
-        if ((rc = mg_iterate<TI>(c, nbr, ctl, first))) return rc;
+        if ((rc = mg_iterate<TI>(c, sys, nbr, ctl, first))) return rc;
         const int d = c->sc_host->done;
         if (d == DONE_FAILED) return fail(c, ISPH_E_NOCONV, "multigrid PPE solve did not converge");
         if (d != DONE_UPDATE) return fail(c, ISPH_E_STATE, "unexpected solver state");
@@ -805,6 +828,10 @@ static int solve_mg_t(isph_ctx* c, const TI* nbr) {
 }
 
 int solve_mg(isph_ctx* c) {
+    return solve_mg_on(c, MGSystem{c->val64, c->val32, c->rsum, c->rhs, c->v64[8]});
+}
+
+int solve_mg_on(isph_ctx* c, const MGSystem& sys) {
     k_to_short<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, (size_t)c->cap, c->nbr, c->cnt, c->nbr16,
                                                      c->err);
     uint32_t e = 0;
@@ -813,7 +840,7 @@ int solve_mg(isph_ctx* c) {
     const bool narrow = !(e & ERR_INDEX16);
     if (!narrow) HIPCHK(c, hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->stream));
     c->narrow_idx = narrow;
-    return narrow ? solve_mg_t<short>(c, c->nbr16) : solve_mg_t<int>(c, c->nbr);
+    return narrow ? solve_mg_t<short>(c, sys, c->nbr16) : solve_mg_t<int>(c, sys, c->nbr);
 }
 
 }  // namespace host

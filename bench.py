@@ -317,14 +317,19 @@ def run_reference_scenario(steps, cpu_steps):
     return out
 
 
-def run_adjoint(dx, nsteps, every):
+def run_adjoint(dx, nsteps, every, multigrid=False):
     """Adjoint/forward cost on the porous beach-slope dam break: nsteps FP64
     steps with checkpoints every `every` steps, objective = time-mean
     Shepard pressure at 3 probes on the slope face + water depth at
-    x = 1.2 m and x = 0.3 m; gradient with respect to (por, alpha, beta)."""
+    x = 1.2 m and x = 0.3 m; gradient with respect to (por, alpha, beta).
+    With `multigrid` both the forward and the transposed solves are
+    multigrid-preconditioned (to 1e-9)."""
     import torch
     from beach_slope_confined_amd import adjoint, capi, configs
+    from beach_slope_confined_amd.params import PREC_MIXED_MG
     cfg = configs.dam_break(dx)
+    if multigrid:
+        cfg["params"] = cfg["params"].copy(precision=PREC_MIXED_MG, tol=1e-9)
     sim = configs.make_sim(cfg, capi.IsphSim)
     sim.checkpoint(0)
     obj = adjoint.slope_objective(gauges=(1.2, 0.3))
@@ -348,8 +353,10 @@ def run_adjoint(dx, nsteps, every):
     bwd = time.perf_counter() - a
     n = len(cfg["parts"]["x"])
     sim.close()
+    kind = "multigrid" if multigrid else "FP64"
     return {"workload": f"dam_break dx={dx:g}: {cfg['n_fluid']} fluid, {n} particles, "
-                        f"{len(cfg['soil']['x'])} soil; {nsteps} FP64 steps, checkpoints every {every}",
+                        f"{len(cfg['soil']['x'])} soil; {nsteps} {kind} steps, checkpoints "
+                        f"every {every}",
             "forward_s": round(fwd, 4), "plain_forward_s": round(plain, 4),
             "backward_s": round(bwd, 4), "adjoint_over_forward": round(bwd / fwd, 3),
             "J": J, "grad_por_alpha_beta": [float(v) for v in g],
@@ -625,6 +632,12 @@ def main():
     if not args.no_adjoint:
         try:
             line["adjoint"] = run_adjoint(args.adjoint_dx, args.adjoint_steps, 10)
+            mg = run_adjoint(args.adjoint_dx, args.adjoint_steps, 10, multigrid=True)
+            g64 = np.array(line["adjoint"]["grad_por_alpha_beta"])
+            mg["grad_rel_diff_vs_fp64"] = [float(v) for v in
+                                           np.abs(np.array(mg["grad_por_alpha_beta"]) - g64) /
+                                           np.abs(g64)]
+            line["adjoint"]["multigrid"] = mg
         except Exception as e:  # reported, not hidden
             line["adjoint"] = {"error": str(e)}
 
