@@ -603,7 +603,25 @@ def main():
             big = configs.beach_large()
             rbig = run_gpu(big, args.steps, args.warmup, PREC_MIXED_MG, False, tol=1e-9,
                            lists=False)
-            rbig["sim"].close()
+            # the adjoint of one multigrid step (the configuration does not
+            # survive a second step at this resolution): probes on this
+            # tank's slope face y = (x - toe)/3, gauge in the column
+            import torch
+            from beach_slope_confined_amd import adjoint as adj
+            sb = rbig["sim"]
+            toe = 0.5 * 2048 * 0.0005
+            obj = adj.slope_objective(toe=toe, probe_x=(toe + 0.05, toe + 0.1, toe + 0.15),
+                                      gauges=(0.5,))
+            sb.restore(0)
+            torch.cuda.synchronize()
+            a0 = time.perf_counter()
+            Jb = sb.adjoint_forward(obj, 1, 1)
+            torch.cuda.synchronize()
+            a1 = time.perf_counter()
+            gb, _, itb = sb.adjoint_backward()
+            torch.cuda.synchronize()
+            a2 = time.perf_counter()
+            sb.close()
             nfb = big["n_fluid"]
             line["large_mg"] = {
                 "workload": big["name"], "n_fluid": nfb, "n_total": len(big["parts"]["x"]),
@@ -611,6 +629,10 @@ def main():
                 "value": round(nfb * args.steps / rbig["seconds"], 1),
                 "ms_per_step": round(1e3 * rbig["seconds"] / args.steps, 3), "tol": 1e-9,
                 "iters_per_step": rbig["iters"][args.warmup:], "relres": rbig["relres"],
+                "adjoint_one_step": {"forward_s": round(a1 - a0, 4), "backward_s": round(a2 - a1, 4),
+                                     "adjoint_over_forward": round((a2 - a1) / (a1 - a0), 3),
+                                     "J": Jb, "grad_por_alpha_beta": [float(v) for v in gb],
+                                     "transposed_iters": int(itb)},
             }
             del big
         except Exception as e:  # reported, not hidden
