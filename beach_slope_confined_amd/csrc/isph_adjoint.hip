@@ -35,6 +35,8 @@ namespace host {
 
 struct Ckpt {
     StateView v{};
+    int* key = nullptr;   // the sorted bin keys and bin starts of the state
+    int* bins = nullptr;
     SimScalars ss{};
     int n = 0;
 };
@@ -79,13 +81,20 @@ namespace {
 
 int alloc_ckpt(isph_ctx* c, Ckpt& k) {
     if (k.v.pos) return ISPH_OK;
-    return alloc_state(c, k.v, (size_t)c->cap);
+    int rc = alloc_state(c, k.v, (size_t)c->cap);
+    rc |= dalloc(c, &k.key, (size_t)c->cap);
+    rc |= dalloc(c, &k.bins, (size_t)c->grid.nbins + 2);
+    return rc;
 }
 
 int save_ckpt(isph_ctx* c, Ckpt& k) {
     int rc = alloc_ckpt(c, k);
     if (rc) return fail(c, ISPH_E_HIP, "adjoint checkpoint allocation failed");
     if ((rc = copy_state(c, c->st[c->cur], k.v, c->n))) return rc;
+    HIPCHK(c, hipMemcpyAsync(k.key, c->key_s, sizeof(int) * c->n, hipMemcpyDeviceToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(k.bins, c->bincnt, sizeof(int) * (c->grid.nbins + 2),
+                             hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(&k.ss, c->ss, sizeof(SimScalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     k.n = c->n;
@@ -95,6 +104,10 @@ int save_ckpt(isph_ctx* c, Ckpt& k) {
 int load_ckpt(isph_ctx* c, const Ckpt& k) {
     int rc = copy_state(c, k.v, c->st[c->cur], k.n);
     if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->key_s, k.key, sizeof(int) * k.n, hipMemcpyDeviceToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->bincnt, k.bins, sizeof(int) * (c->grid.nbins + 2),
+                             hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->ss, &k.ss, sizeof(SimScalars), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->n = k.n;
@@ -105,7 +118,9 @@ int load_ckpt(isph_ctx* c, const Ckpt& k) {
 
 void free_ckpt(Ckpt& k) {
     if (k.v.pos) free_state(k.v);
-    k.v = StateView{};
+    if (k.key) (void)hipFree(k.key);
+    if (k.bins) (void)hipFree(k.bins);
+    k = Ckpt{};
 }
 
 int alloc_tape(isph_ctx* c, StepTape& t) {

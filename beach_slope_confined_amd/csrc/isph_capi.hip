@@ -798,6 +798,7 @@ int isph_destroy(isph_ctx* c) {
         free_state(sl.v);
         if (sl.key) (void)hipFree(sl.key);
         if (sl.fsol) (void)hipFree(sl.fsol);
+        if (sl.bins) (void)hipFree(sl.bins);
     }
     for (int k = 0; k < PROF_SLOTS; ++k)
         for (auto& e : c->prof.pool[k]) {
@@ -1196,12 +1197,15 @@ int isph_checkpoint(isph_ctx* c, int32_t slot) {
         if (alloc_state(c, sl.v, (size_t)c->cap)) return fail(c, ISPH_E_HIP, "checkpoint allocation failed");
         HIPCHK(c, hipMalloc((void**)&sl.key, sizeof(int) * c->cap));
         HIPCHK(c, hipMalloc((void**)&sl.fsol, sizeof(double) * c->cap));
+        HIPCHK(c, hipMalloc((void**)&sl.bins, sizeof(int) * (c->grid.nbins + 2)));
     }
     int rc = copy_state(c, c->st[c->cur], sl.v, c->n);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(sl.key, c->key_s, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(sl.fsol, c->fsol, sizeof(double) * c->cap, hipMemcpyDeviceToDevice,
                              c->stream));
+    HIPCHK(c, hipMemcpyAsync(sl.bins, c->bincnt, sizeof(int) * (c->grid.nbins + 2),
+                             hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(&sl.ss, c->ss, sizeof(SimScalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     sl.n = c->n;
@@ -1218,6 +1222,8 @@ int isph_restore(isph_ctx* c, int32_t slot) {
     HIPCHK(c, hipMemcpyAsync(c->key_s, sl.key, sizeof(int) * sl.n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->fsol, sl.fsol, sizeof(double) * c->cap, hipMemcpyDeviceToDevice,
                              c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->bincnt, sl.bins, sizeof(int) * (c->grid.nbins + 2),
+                             hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->ss, &sl.ss, sizeof(SimScalars), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->n = sl.n;

@@ -98,6 +98,7 @@ struct isph_ctx {
         int* key = nullptr;
         isph::SimScalars ss{};
         double* fsol = nullptr;
+        int* bins = nullptr;  // bin starts of the last sort (the neighbour search reads them)
         int n = 0;
         bool used = false;
     } slots[64];
