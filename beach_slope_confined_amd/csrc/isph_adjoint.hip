@@ -403,10 +403,11 @@ int adj_ppe(isph_ctx* c, const AdjView& a, const StateView& s, int* nbr, int* cn
     if (!rc && pr.precision == ISPH_PREC_MIXED_MG) {
         // the forward solves with the multigrid preconditioner; so does the
         // transposed solve, on A^T's own Galerkin hierarchy
-        const size_t N = (size_t)c->cap, M = (size_t)pr.max_nbr;
-        if (!S->val32T && dalloc(c, &S->val32T, N * M)) S->val32T = nullptr;
-        if (!S->rsumT && dalloc(c, &S->rsumT, N)) S->rsumT = nullptr;
-        if (!S->val32T || !S->rsumT) rc = fail(c, ISPH_E_HIP, "adjoint buffer allocation failed");
+        if (!S->val32T) {
+            const size_t N = (size_t)c->cap, M = (size_t)pr.max_nbr;
+            if (dalloc(c, &S->val32T, N * M) || dalloc(c, &S->rsumT, N))
+                rc = fail(c, ISPH_E_HIP, "adjoint buffer allocation failed");
+        }
         if (!rc) {
             // the hierarchy reads the bin of every row: the taped state is
             // sorted by the keys of its positions, which have not moved

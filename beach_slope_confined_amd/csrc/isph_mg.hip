@@ -126,15 +126,12 @@ __global__ void k_mg_galerkin(int nx, int ny, const float* __restrict__ A, int n
 // MODE 0: b = p = r + beta q (r on a restart), from rq = (r, q);
 // MODE 1: b = s = r - alpha v, from rv = (r, v).  b goes to bout (FP64,
 // the outer recurrences need it); x0 = omega b (Jacobi from 0, unit
-// diagonal) and r0 = b - A x0 = b - omega A b.  The neighbours' b is formed
-// in FP32 from the FP32 copies of the pairs (half the gathered bytes); the
-// smoother is FP32 either way.
+// diagonal) and r0 = b - A x0 = b - omega A b.
 template <class TI, int MODE, bool FIRST>
 __global__ void __launch_bounds__(BLOCK) k_mg_fine_down(int n, size_t ld, const TI* __restrict__ nbr,
                                                         const int* __restrict__ cnt,
                                                         const float* __restrict__ val,
                                                         const double2* __restrict__ pairs,
-                                                        const float2* __restrict__ pairs32,
                                                         const SolverScalars* sc,
                                                         double* __restrict__ bout,
                                                         float* __restrict__ x0,
@@ -143,31 +140,22 @@ __global__ void __launch_bounds__(BLOCK) k_mg_fine_down(int n, size_t ld, const 
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const double coef = MODE == 0 ? sc->betas : sc->alpha;
-    const float fcoef = (float)coef;
-    const double2 ui = pairs[i];
-    const double bi = MODE == 0 ? (FIRST ? ui.x : ui.x + coef * ui.y) : ui.x - coef * ui.y;
-    auto bj = [&](int j) -> float {
-        const float2 u = pairs32[j];
-        if (MODE == 0) return FIRST ? u.x : u.x + fcoef * u.y;
-        return u.x - fcoef * u.y;
+    auto bj = [&](int j) -> double {
+        const double2 u = pairs[j];
+        if (MODE == 0) return FIRST ? u.x : u.x + coef * u.y;
+        return u.x - coef * u.y;
     };
+    const double bi = bj(i);
     bout[i] = bi;
     const float fb = (float)bi;
     float acc = fb;
     const int c = cnt[i];
     for (int k = 0; k < c; ++k) {
         const size_t o = (size_t)k * ld + i;
-        acc += val[o] * bj(mg_col(nbr[o], i));
+        acc += val[o] * (float)bj(mg_col(nbr[o], i));
     }
     x0[i] = MG_OMEGA * fb;
     r0[i] = fb - MG_OMEGA * acc;
-}
-
-// FP32 copy of the (r, q) pairs after a reliable update or a restart
-// rewrote them
-__global__ void k_mg_pairs32(int n, const double2* __restrict__ rq, float2* __restrict__ rq32) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) rq32[i] = make_float2((float)rq[i].x, (float)rq[i].y);
 }
 
 // restriction to the cells: b1[c] = sum of r0 over the cell's particles
@@ -507,8 +495,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_v(int n, size_t ld, const TI* __re
                                                 const double* __restrict__ ph,
                                                 const double2* __restrict__ rq,
                                                 const double* __restrict__ rhat,
-                                                double2* __restrict__ rv, float2* __restrict__ rv32,
-                                                double* __restrict__ part,
+                                                double2* __restrict__ rv, double* __restrict__ part,
                                                 const SolverScalars* sc) {
     if (sc->done) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -516,9 +503,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_v(int n, size_t ld, const TI* __re
     double v = 0.0;
     if (i < n) {
         const double s = mg_spmv<TI>(i, cnt[i], rsum[i], nbr, val, ld, ph);
-        const double r = rq[i].x;
-        rv[i] = make_double2(r, s);
-        rv32[i] = make_float2((float)r, (float)s);
+        rv[i] = make_double2(rq[i].x, s);
         v = s * rhat[i];
     }
     v = block_sum(v);
@@ -562,7 +547,6 @@ __global__ void __launch_bounds__(BLOCK) k_mg_x(int n, const double* __restrict_
                                                 const double2* __restrict__ rv,
                                                 const double* __restrict__ rhat,
                                                 double* __restrict__ xs, double2* __restrict__ rq,
-                                                float2* __restrict__ rq32,
                                                 double* __restrict__ part, const SolverScalars* sc) {
     if (sc->done) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -574,7 +558,6 @@ __global__ void __launch_bounds__(BLOCK) k_mg_x(int n, const double* __restrict_
         const double r = s[i] - w * t[i];
         const double q = p[i] - w * rv[i].y;
         rq[i] = make_double2(r, q);
-        rq32[i] = make_float2((float)r, (float)q);
         v0 = r * rhat[i];
         v1 = r * r;
     }
@@ -719,13 +702,13 @@ static int mg_setup(isph_ctx* c, const MGSystem& sys) {
 // into bout, preconditioned vector into out
 template <class TI, int MODE, bool FIRST>
 static void mg_vcycle(isph_ctx* c, const MGSystem& sys, const TI* nbr, const double2* pairs,
-                      const float2* pairs32, double* bout, double* out) {
+                      double* bout, double* out) {
     MGData* m = c->mg;
     const GridInfo& g = c->grid;
     const int n = c->n;
     const size_t ld = (size_t)c->cap;
     k_mg_fine_down<TI, MODE, FIRST><<<nblk(n), BLOCK, 0, c->stream>>>(
-        n, ld, nbr, c->cnt, sys.val32, pairs, pairs32, c->sc, bout, m->x0, m->r0);
+        n, ld, nbr, c->cnt, sys.val32, pairs, c->sc, bout, m->x0, m->r0);
     k_mg_restrict1<<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(g, c->bincnt, m->r0,
                                                                          m->lv[0].b, c->sc);
     const int lb = m->lb;
@@ -768,19 +751,18 @@ static int mg_iterate(isph_ctx* c, const MGSystem& sys, const TI* nbr, const Sol
         }
         for (int b = 0; b < batch; ++b) {
             if (first)
-                mg_vcycle<TI, 0, true>(c, sys, nbr, c->rq64, c->rq32, p, ph);
+                mg_vcycle<TI, 0, true>(c, sys, nbr, c->rq64, p, ph);
             else
-                mg_vcycle<TI, 0, false>(c, sys, nbr, c->rq64, c->rq32, p, ph);
+                mg_vcycle<TI, 0, false>(c, sys, nbr, c->rq64, p, ph);
             k_mg_v<TI><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, sys.val32, sys.rsum, ph,
-                                                       c->rq64, rhat, c->rv64, c->rv32, c->part,
-                                                       c->sc);
+                                                       c->rq64, rhat, c->rv64, c->part, c->sc);
             k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_ALPHA, grid, c->part, c->sc, ctl);
-            mg_vcycle<TI, 1, false>(c, sys, nbr, c->rv64, c->rv32, s, sh);
+            mg_vcycle<TI, 1, false>(c, sys, nbr, c->rv64, s, sh);
             k_mg_t<TI><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, sys.val32, sys.rsum, sh, s,
                                                        t, c->part, c->sc);
             k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_OMEGA, grid, c->part, c->sc, ctl);
             k_mg_x<<<grid, BLOCK, 0, c->stream>>>(n, p, ph, s, sh, t, c->rv64, rhat, xs, c->rq64,
-                                                   c->rq32, c->part, c->sc);
+                                                   c->part, c->sc);
             k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_BETA, grid, c->part, c->sc, ctl);
             first = false;
         }
@@ -813,7 +795,6 @@ static int solve_mg_t(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
     if (c->sc_host->done == DONE_CONVERGED) return ISPH_OK;
     k_mx_start<double><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, c->rq64, c->v64[0], c->v64[4],
                                                       c->part, c->sc);
-    k_mg_pairs32<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->rq64, c->rq32);
     k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_START, grid, c->part, c->sc, ctl);
     HIPCHK(c, hipGetLastError());
     if ((rc = read_sc(c))) return rc;
@@ -841,7 +822,6 @@ static int solve_mg_t(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
                                                                 c->part, c->sc);
             k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_REPLACE, grid, c->part, c->sc, ctl);
         }
-        k_mg_pairs32<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->rq64, c->rq32);
         HIPCHK(c, hipGetLastError());
         if ((rc = read_sc(c))) return rc;
     }
