@@ -408,8 +408,9 @@ def main():
     global args
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(the multi-GPU decomposition is described in DESIGN.md)"}))
+        print(json.dumps({"error": "the ISPH library runs on one GPU per process; --gpus must be "
+                          "1 (the step shards by x-slabs of cells, each GPU a slab with a "
+                          "one-cell halo, but no multi-GPU path is built)"}))
         sys.exit(2)
     import torch
     torch.cuda.init()
