@@ -300,10 +300,17 @@ def run_reference_scenario(steps, cpu_steps):
     gpu = capi.IsphSim(sc["params"], cap, len(sc["soil"]["x"]))
     g_ms, g_it = timed(gpu, steps)
     gpu.close()
+    # the same run with the multigrid-preconditioned PPE (to 1e-9)
+    from beach_slope_confined_amd.params import PREC_MIXED_MG
+    mg = capi.IsphSim(sc["params"].copy(precision=PREC_MIXED_MG, tol=1e-9), cap,
+                      len(sc["soil"]["x"]))
+    m_ms, m_it = timed(mg, steps)
+    mg.close()
     out = {"workload": "reference scenario (main.f90 phases 1-3, buffers, salinity): "
                        f"{sc['n_fluid']} fluid, {len(sc['parts']['x'])} particles, "
                        f"{len(sc['soil']['x'])} soil", "steps_per_phase": steps,
-           "gpu_ms_per_step": g_ms, "gpu_ppe_iters_last_step": g_it}
+           "gpu_ms_per_step": g_ms, "gpu_ppe_iters_last_step": g_it,
+           "gpu_multigrid_ms_per_step": m_ms, "gpu_multigrid_ppe_iters_last_step": m_it}
     if cpu_steps:
         from oracle import pyoracle
         from oracle.pyoracle import OracleSim
