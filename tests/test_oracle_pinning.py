@@ -389,6 +389,64 @@ def test_int_vel_uniform_flow_is_gravity_only():
     assert np.allclose(vys[inner], -0.05 + (-9.81) * 1e-4, rtol=1e-14)
 
 
+@pytest.mark.parametrize("kernel", [KERNEL_HYBRID, KERNEL_WENDLAND_C2])
+def test_int_vel_viscous_term_is_the_laplacian(kernel):
+    """Laminar v* for the quadratic shear vx = 3 y^2 on a lattice interior:
+    the pair form 2 V (v_i - v_j)(dx . L gradW)/(r^2 + eta) times the
+    harmonic-mean viscosity (integrator.f90:532-541) is nu * laplacian(v),
+    here 6 nu in x and 0 in y, to the eta = 1e-3 h^2 regularisation
+    (r^2/(r^2 + eta) = 1 - 1.7e-3 at the nearest neighbours)."""
+    nside = 31
+    parts = lattice(nside)
+    parts["vx"] = 3.0 * parts["y"] ** 2
+    prm = lattice_params(kernel=kernel)
+    sim = sim_from(parts, prm)
+    sim.neighbour()
+    sim.effpor()
+    sim.int_vel()  # no eddyvis: nut = tke = 0, the laminar operator alone
+    p = sim.get_particles()
+    inner = interior(p, nside, margin=5)
+    lap = (sim.get_field("vxs")[inner] - p["vx"][inner]) / (prm.dt * prm.nu)
+    assert np.all(np.abs(lap - 6.0) <= 3e-3 * 6.0), (lap.min(), lap.max())
+    assert lap.max() - lap.min() <= 1e-6
+    assert np.allclose(sim.get_field("vys")[inner], -9.81 * prm.dt, rtol=1e-14, atol=0.0)
+
+
+def test_int_vel_pair_viscosity_is_the_harmonic_mean():
+    """Two fluid particles, reference viscosity law nu(oden) = mu(S)/oden
+    with S = oden - 1000 (porous.f90:21-26), no free surface (lamfs = 0), no
+    eddy viscosity, no soil: the viscous term of particle 0 is one pair term
+    times the pair viscosity.  Changing the neighbour's oden (with its mass,
+    so that its volume m/rho, rho = n oden, stays) scales it by the harmonic
+    mean 2 nu_b / (nu_a + nu_b) over nu_a, integrator.f90:532-541."""
+    def mu(oden):
+        sp = oden - 1000.0
+        return ((1.002 + 0.001652 * sp) + 0.0000083 * (sp * sp)) * 0.001
+
+    def viscous(oden_j):
+        parts = lattice(1)
+        parts = {k: np.concatenate([v, v]) for k, v in parts.items()}
+        parts["x"][1] += DX
+        parts["y"][1] += 0.3 * DX
+        parts["vx"][1] = 0.1  # particle 0 at rest: v* - v has no cancellation
+        parts["oden"][1] = oden_j
+        parts["mass"][1] *= oden_j / 1000.0  # effpor sets rho = n oden: same volume m/rho
+        parts["pid"] = np.array([0, 1], np.int32)
+        prm = lattice_params(visc_law=VISC_REFERENCE, lamfs=0.0)
+        sim = sim_from(parts, prm)
+        sim.neighbour()
+        sim.effpor()
+        sim.int_vel()
+        p = sim.get_particles()
+        i = int(np.flatnonzero(p["pid"] == 0)[0])
+        return (sim.get_field("vxs")[i] - p["vx"][i]) / prm.dt
+
+    nu_a, nu_b = mu(1000.0) / 1000.0, mu(1025.0) / 1025.0
+    ratio = viscous(1025.0) / viscous(1000.0)
+    assert ratio == pytest.approx(2.0 * nu_b / (nu_a + nu_b), rel=1e-12)
+    assert abs(ratio - (nu_a + nu_b) / (2.0 * nu_a)) > 1e-6  # not the arithmetic mean
+
+
 def test_shift_zero_on_lattice_and_restoring_for_a_displaced_particle():
     nside = 21
     parts = lattice(nside)
