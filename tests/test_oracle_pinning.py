@@ -17,7 +17,8 @@ from scipy import integrate
 
 from beach_slope_confined_amd import configs
 from beach_slope_confined_amd.params import (DRAG_ERGUN, DRAG_REFERENCE, KERNEL_HYBRID,
-                                             KERNEL_WENDLAND_C2, VISC_REFERENCE, make_params)
+                                             FLAG_FREE, KERNEL_WENDLAND_C2, VISC_REFERENCE,
+                                             make_params)
 from oracle import pyoracle
 from oracle.pyoracle import OracleSim
 
@@ -289,6 +290,53 @@ def test_ppe_rhs_zero_for_uniform_velocity():
     assert np.array_equal(rhs[interior(sim.get_particles(), nside, 0)], np.zeros(n))
 
 
+def test_ppe_wall_rows_couple_symmetrically():
+    """The wall rows (isph.f90:214-421).  A dummy wall (tid 2) row couples to
+    walls only; a fixed wall (tid 1) row takes a fluid neighbour's entry from
+    that neighbour's own corrected gradient L_j grad W and the wall's volume
+    m_i / rho_i, which is exactly the fluid row's entry for the wall, and its
+    wall entries from the same uncorrected form as the dummy rows.  So
+    before the Jacobi scaling the wall-fluid (tid 1) and wall-wall couplings
+    are symmetric, and after it (S = D^-1 A, any diagonal D) every cycle of
+    such couplings satisfies S_ij S_jk S_ki = S_ji S_kj S_ik (Kolmogorov's
+    criterion).  Wall volumes are made 1.3x the fluid's so that a row built
+    with the neighbour's volume, or the wall's own L, breaks it."""
+    cfg = configs.dam_break(0.01)
+    wall = cfg["parts"]["tid"] < 3
+    cfg["parts"]["mass"][wall] *= 1.3
+    sim = assembled(cfg)
+    rp, col, val, _ = sim.get_csr()
+    tid = sim.get_particles()["tid"]
+    n = len(tid)
+    S = sp.csr_matrix((val, col, rp), shape=(n, n)).todok()
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    ti, tj = tid[rows], tid[col]
+    # dummy walls see walls only; fluid rows see every neighbour
+    assert np.all(val[(ti == 2) & (tj == 3)] == 0.0)
+    assert np.all(val[(ti == 3) & (rows != col)] != 0.0)
+
+    def sym(a, b):
+        return (tid[a] == 1 and tid[b] == 3) or (tid[a] == 3 and tid[b] == 1) or \
+            (tid[a] <= 2 and tid[b] <= 2)
+
+    adj = [set() for _ in range(n)]
+    for a, b in zip(rows, col):
+        if a != b and sym(a, b):
+            adj[a].add(b)
+    checked = {"wf": 0, "ww": 0}
+    for i in np.flatnonzero(tid == 1):
+        for j in adj[i]:
+            assert i in adj[j] and S[i, j] != 0.0 and S[j, i] != 0.0
+            for k in adj[i] & adj[j]:
+                if k <= j:
+                    continue
+                fwd = S[i, j] * S[j, k] * S[k, i]
+                bwd = S[j, i] * S[k, j] * S[i, k]
+                assert abs(fwd - bwd) <= 1e-12 * abs(fwd), (i, j, k, fwd, bwd)
+                checked["wf" if 3 in (tid[j], tid[k]) else "ww"] += 1
+    assert checked["wf"] > 500 and checked["ww"] > 500, checked
+
+
 # ---------------------------------------------------------------- solver --
 def test_bicgstab_known_answer_solvertest():
     with open(os.path.join(GOLDEN, "solvertest_5x5.json")) as f:
@@ -445,6 +493,163 @@ def test_int_vel_pair_viscosity_is_the_harmonic_mean():
     ratio = viscous(1025.0) / viscous(1000.0)
     assert ratio == pytest.approx(2.0 * nu_b / (nu_a + nu_b), rel=1e-12)
     assert abs(ratio - (nu_a + nu_b) / (2.0 * nu_a)) > 1e-6  # not the arithmetic mean
+
+
+def jittered_block(nside, seed, amp=0.15):
+    parts = lattice(nside)
+    rng = np.random.default_rng(seed)
+    parts["x"] = parts["x"] + rng.uniform(-amp, amp, parts["x"].size) * DX
+    parts["y"] = parts["y"] + rng.uniform(-amp, amp, parts["y"].size) * DX
+    return parts
+
+
+def linear_velocity(parts, a, b, c, d):
+    parts["vx"] = 0.3 + a * parts["x"] + c * parts["y"]
+    parts["vy"] = -0.1 + d * parts["x"] + b * parts["y"]
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_free_flag_is_the_trace_threshold_on_irregular_blocks(seed):
+    """nsearch2 (search.f90:270,296-298): divr = tr(A) of the moment matrix
+    A = sum V_j (x_j - x_i) (x) grad_i W that the KGC matrix inverts, and a
+    fluid particle is free iff divr <= lamfs * maxdivr (0.8 * 2).  On a
+    jittered block A = L^-1 (L pinned by the exact gradients below), the
+    interior reaches tr(A) = 2 to the jitter's error and every particle on the block's edge rows,
+    with half a support, falls below the threshold."""
+    nside = 25
+    parts = jittered_block(nside, seed)
+    parts["flags"][:] = FLAG_FREE  # a stale flag must be cleared
+    prm = lattice_params()
+    sim = sim_from(parts, prm)
+    sim.neighbour()
+    p = sim.get_particles()
+    L = sim.get_field("coff")
+    divr = sim.get_field("divr")
+    A = np.linalg.inv(L.reshape(-1, 2, 2))
+    assert np.allclose(divr, A[:, 0, 0] + A[:, 1, 1], rtol=1e-12, atol=0.0)
+    free = (p["flags"] & FLAG_FREE) != 0
+    assert np.array_equal(free, divr <= prm.lamfs * prm.maxdivr)
+    inner = interior(p, nside, margin=3)
+    assert np.all(np.abs(divr[inner] - 2.0) < 0.25)  # jitter 0.15 dx
+    ii, jj = np.divmod(p["pid"], nside)
+    edge = (ii == 0) | (jj == 0) | (ii == nside - 1) | (jj == nside - 1)
+    assert np.all(free[edge]) and not np.any(free[inner])
+
+
+@pytest.mark.parametrize("seed", [0, 3])
+def test_free_surface_artificial_viscosity_closed_form(seed):
+    """varts of a free particle (integrator.f90:475-500) is
+    dl^2 |(d(vx/n)/dx, d(vy/n)/dy)| |v_i| / ((umax + 1e-6) n_i), with the
+    corrected gradient of v/n.  For a linear velocity field the corrected
+    gradient is exact at any particle arrangement (A is symmetric for a
+    radial kernel, so L A^T = I), so on a jittered block with uniform
+    porosity n: varts = dl^2 sqrt(a^2 + b^2) / n^2 |v_i| / (umax + 1e-6) for
+    v = (a x + c y, d x + b y) + const; the shear c, d does not enter."""
+    nside = 21
+    parts = jittered_block(nside, seed)
+    a, b, c, d = 0.7, -0.4, 1.3, -2.1
+    linear_velocity(parts, a, b, c, d)
+    prm = lattice_params()
+    sim = sim_from(parts, prm)
+    sim.neighbour()
+    sim.effpor()
+    n = 0.5
+    sim.set_field("porosity", np.full(sim.count(), n))
+    umax = 0.37
+    sim.set_scalars(umax, prm.dt, 0.0)
+    sim.int_vel()
+    p = sim.get_particles()
+    free = (p["flags"] & FLAG_FREE) != 0
+    assert free.sum() >= 4 * (nside - 1)
+    speed = np.hypot(p["vx"], p["vy"])
+    want = DX * DX * math.hypot(a / n, b / n) * speed / ((umax + 1e-6) * n)
+    got = sim.get_field("varts")
+    assert np.allclose(got[free], want[free], rtol=1e-9, atol=0.0)
+    assert np.all(got[~free] == 0.0)
+
+
+def test_int_vel_tke_gradient_is_two_thirds_n_grad_k():
+    """The SGS kinetic-energy term (integrator.f90:640-660): v* - v =
+    -dt (2/3) n_i grad k with the corrected gradient, so a linear
+    k = c x + d y at porosity n gives -dt (2/3) n (c, d) exactly (plus g n dt
+    in y) on a still fluid block, edges included."""
+    nside = 21
+    parts = jittered_block(nside, 5)
+    prm = lattice_params()
+    sim = sim_from(parts, prm)
+    sim.neighbour()
+    sim.effpor()
+    n = 0.6
+    sim.set_field("porosity", np.full(sim.count(), n))
+    cx, cy = 3.0e-3, -1.7e-3
+    p = sim.get_particles()
+    sim.set_field("tke", 1e-4 + cx * p["x"] + cy * p["y"])
+    sim.int_vel()
+    vxs, vys = sim.get_field("vxs"), sim.get_field("vys")
+    assert np.allclose(vxs, -prm.dt * (2.0 / 3.0) * n * cx, rtol=1e-9, atol=0.0)
+    assert np.allclose(vys, prm.dt * (-(2.0 / 3.0) * n * cy + prm.g * n), rtol=1e-9, atol=0.0)
+
+
+@pytest.mark.parametrize("nut_t", [2e-5, 4e-7])
+def test_int_vel_uniform_eddy_viscosity_adds_to_the_laminar(nut_t):
+    """A uniform eddy viscosity nu_t between fluid particles enters the
+    viscous pair term beside the laminar one: the harmonic mean of equal
+    values above 1e-6 and their arithmetic mean below it are both nu_t, so
+    the quadratic shear's viscous term scales by (nu + nu_t)/nu exactly."""
+    nside = 31
+
+    def viscous(nut):
+        parts = lattice(nside)
+        parts["vx"] = 3.0 * parts["y"] ** 2
+        prm = lattice_params()
+        sim = sim_from(parts, prm)
+        sim.neighbour()
+        sim.effpor()
+        sim.set_field("nut", np.full(sim.count(), nut))
+        sim.int_vel()
+        p = sim.get_particles()
+        inner = interior(p, nside, margin=5)
+        return (sim.get_field("vxs")[inner] - p["vx"][inner]) / prm.dt, prm.nu
+
+    v0, nu = viscous(0.0)
+    vt, _ = viscous(nut_t)
+    # v* - v = 6e-10 against |v| ~ 1: the ratio holds to the cancellation, 1e-7
+    assert np.allclose(vt / v0, (nu + nut_t) / nu, rtol=1e-6)
+
+
+@pytest.mark.parametrize("nut_a,nut_b,mean", [
+    (1e-3, 3e-3, 1.5e-3),      # both above 1e-6: harmonic, 2ab/(a+b)
+    (1e-7, 5e-7, 3e-7),        # both below: arithmetic, (a+b)/2
+    (2e-7, 4e-3, 2 * 2e-7 * 4e-3 / (2e-7 + 4e-3)),  # one above: harmonic
+])
+def test_int_vel_pair_eddy_viscosity_mean(nut_a, nut_b, mean):
+    """integrator.f90:572-584: two fluid particles with no free surface; the
+    viscous term of particle 0 is one pair term times nu + the pair eddy
+    viscosity.  Less the term at nu_t = 0 (the arithmetic mean of zeros), it
+    scales by mean / 1e-3 against a pair with equal nu_t = 1e-3.  (nu = 0
+    would make the laminar harmonic mean 0/0.)"""
+    def viscous(na, nb):
+        parts = lattice(1)
+        parts = {k: np.concatenate([v, v]) for k, v in parts.items()}
+        parts["x"][1] += DX
+        parts["y"][1] += 0.3 * DX
+        parts["vx"][1] = 0.1
+        parts["pid"] = np.array([0, 1], np.int32)
+        prm = lattice_params(lamfs=0.0)
+        sim = sim_from(parts, prm)
+        sim.neighbour()
+        sim.effpor()
+        p = sim.get_particles()
+        i = int(np.flatnonzero(p["pid"] == 0)[0])
+        nut = np.where(p["pid"] == 0, na, nb)
+        sim.set_field("nut", nut)
+        sim.int_vel()
+        return (sim.get_field("vxs")[i] - p["vx"][i]) / prm.dt
+
+    lam = viscous(0.0, 0.0)
+    ref = viscous(1e-3, 1e-3) - lam
+    assert lam != 0.0 and ref != 0.0
+    assert (viscous(nut_a, nut_b) - lam) / ref == pytest.approx(mean / 1e-3, rel=1e-10)
 
 
 def test_shift_zero_on_lattice_and_restoring_for_a_displaced_particle():
