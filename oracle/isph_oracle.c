@@ -31,8 +31,6 @@
  * Parity unpinned: tests/test_oracle_pinning.py and its neighbours check
  * every routine against closed forms, brute force, scipy or the reference's
  * known answers, except these parts, which only agree with the GPU:
- *   - ora_int_vel: the transfer W(r) varts_j / W0 of a free neighbour's
- *     artificial viscosity to a particle that is not free;
  *   - ora_ppe_assemble: the right-hand side of the fixed-wall (tid 1) rows;
  *   - ora_cellshift: the order of the particles within one cell.
  */

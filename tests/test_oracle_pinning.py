@@ -495,6 +495,47 @@ def test_int_vel_pair_viscosity_is_the_harmonic_mean():
     assert abs(ratio - (nu_a + nu_b) / (2.0 * nu_a)) > 1e-6  # not the arithmetic mean
 
 
+def test_int_vel_free_neighbour_passes_on_its_artificial_viscosity():
+    """integrator.f90:543-553: a particle that is not free adds
+    W(r_ij) varts_j / W(0) of each free neighbour j to the pair viscosity.
+    Three particles on a line, the ends free and the middle not (the
+    threshold lamfs * maxdivr put between their divr), ends moving at u and
+    the middle at rest: by symmetry the middle's viscous term is
+    2 T (nu + W(dx) varts_end / W0) with one pair term T, so two runs at nu
+    and 2 nu give T and leave the transfer W(dx) varts_end / W0, with
+    varts_end as the library computed it (pinned by the closed form above)
+    and W by the kernel tests."""
+    def run(lamfs, nu):
+        parts = lattice(1)
+        parts = {k: np.concatenate([v, v, v]) for k, v in parts.items()}
+        parts["x"] = parts["x"] + np.array([-DX, 0.0, DX])
+        parts["vx"] = np.array([0.1, 0.0, 0.1])
+        parts["pid"] = np.arange(3, dtype=np.int32)
+        prm = lattice_params(lamfs=lamfs, nu=nu)
+        sim = sim_from(parts, prm)
+        sim.neighbour()
+        sim.effpor()
+        sim.set_scalars(0.1, prm.dt, 0.0)
+        sim.int_vel()
+        p = sim.get_particles()
+        order = np.argsort(p["pid"])
+        f = (sim.get_field("vxs")[order] - p["vx"][order]) / prm.dt
+        return (p["flags"][order] & FLAG_FREE) != 0, f, sim.get_field("varts")[order], \
+            sim.get_field("divr")[order], prm
+
+    _, _, _, divr, prm = run(0.8, 1e-6)
+    assert divr[0] == divr[2] < divr[1]  # the ends see the middle and each other
+    lamfs = 0.5 * (divr[0] + divr[1]) / prm.maxdivr
+    free, fa, varts, _, prm = run(lamfs, 1e-6)
+    assert list(free) == [True, False, True]
+    assert varts[1] == 0.0 and varts[0] == varts[2] > 0.0
+    _, fb, _, _, _ = run(lamfs, 2e-6)
+    two_t = (fb[1] - fa[1]) / 1e-6
+    transfer = (fa[1] - two_t * 1e-6) / two_t
+    want = pyoracle.kernel_w(prm, DX) * varts[0] / pyoracle.kernel_w0(prm)
+    assert transfer == pytest.approx(want, rel=1e-8)
+
+
 def jittered_block(nside, seed, amp=0.15):
     parts = lattice(nside)
     rng = np.random.default_rng(seed)
