@@ -28,11 +28,9 @@
  *     stays zero;
  *   - fluid particles skipped by opt2_shift keep xs = old position and jump
  *     by it (part_shift.f90:40,180): their shift is zero.
- * Parity unpinned: tests/test_oracle_pinning.py and its neighbours check
+ * Parity pinning: tests/test_oracle_pinning.py and its neighbours check
  * every routine against closed forms, brute force, scipy or the reference's
- * known answers, except these parts, which only agree with the GPU:
- *   - ora_ppe_assemble: the right-hand side of the fixed-wall (tid 1) rows;
- *   - ora_cellshift: the order of the particles within one cell.
+ * known answers; no routine is left "parity unpinned".
  */
 #include "isph_oracle.h"
 

@@ -17,7 +17,8 @@ from scipy import integrate
 
 from beach_slope_confined_amd import configs
 from beach_slope_confined_amd.params import (DRAG_ERGUN, DRAG_REFERENCE, KERNEL_HYBRID,
-                                             FLAG_FREE, KERNEL_WENDLAND_C2, VISC_REFERENCE,
+                                             FLAG_FREE, KERNEL_WENDLAND_C2, ORDER_REFERENCE,
+                                             VISC_REFERENCE,
                                              make_params)
 from oracle import pyoracle
 from oracle.pyoracle import OracleSim
@@ -290,6 +291,43 @@ def test_ppe_rhs_zero_for_uniform_velocity():
     assert np.array_equal(rhs[interior(sim.get_particles(), nside, 0)], np.zeros(n))
 
 
+def test_ppe_fixed_wall_rhs_mirrors_its_fluid_neighbours():
+    """The fixed-wall row's right-hand side (isph.f90:300-330): its term for
+    a fluid neighbour j, m_i/rho_i (v*_j rho^_j/n_j - v*_i rho^_j/n_i) .
+    L_j grad W_ij / dt, equals fluid row j's term for the wall when n = 1.
+    One fixed-wall particle beside a jittered fluid block, fluid v* uniform
+    (so fluid-fluid terms vanish) and the wall's v* different: unscaled,
+    b_wall is the sum of its fluid neighbours' b_j.  The Jacobi scaling
+    divides row k by D_k, and D_j / D_wall = S_wall,j / S_j,wall (the
+    couplings are symmetric), so rhs_wall = sum_j rhs_j S_wall,j / S_j,wall.
+    The wall's volume differs from the fluid's."""
+    nside = 15
+    parts = jittered_block(nside, 11)
+    wall = lattice(1, x0=0.25 - 0.9 * DX, y0=0.25 + 7.2 * DX, tid=1, rho=1300.0)
+    wall["mass"] = wall["mass"] / 1.3 * 1.1
+    wall["pid"] = np.array([nside * nside], np.int32)
+    parts = {k: np.concatenate([parts[k], wall[k]]) for k in parts}
+    sim = sim_from(parts, lattice_params())
+    sim.neighbour()
+    sim.effpor()
+    p = sim.get_particles()
+    w = int(np.flatnonzero(p["tid"] == 1)[0])
+    vxs = np.where(p["tid"] == 1, -0.5, 0.3)
+    vys = np.where(p["tid"] == 1, 0.7, -0.2)
+    sim.set_field("vxs", vxs)
+    sim.set_field("vys", vys)
+    sim.ppe_assemble()
+    rp, col, val, rhs = sim.get_csr()
+    S = sp.csr_matrix((val, col, rp)).todok()
+    nbrs = [j for j in col[rp[w]:rp[w + 1]] if j != w]
+    assert len(nbrs) >= 4
+    want = sum(rhs[j] * S[w, j] / S[j, w] for j in nbrs)
+    assert rhs[w] != 0.0
+    assert rhs[w] == pytest.approx(want, rel=1e-12)
+    far = np.array([j for j in range(len(rhs)) if j != w and S.get((j, w), 0.0) == 0.0])
+    assert np.all(rhs[far] == 0.0)
+
+
 def test_ppe_wall_rows_couple_symmetrically():
     """The wall rows (isph.f90:214-421).  A dummy wall (tid 2) row couples to
     walls only; a fixed wall (tid 1) row takes a fluid neighbour's entry from
@@ -534,6 +572,73 @@ def test_int_vel_free_neighbour_passes_on_its_artificial_viscosity():
     transfer = (fa[1] - two_t * 1e-6) / two_t
     want = pyoracle.kernel_w(prm, DX) * varts[0] / pyoracle.kernel_w0(prm)
     assert transfer == pytest.approx(want, rel=1e-8)
+
+
+def test_cellshift_reference_order_within_and_across_cells():
+    """Known answer from the reference's text, ORDER_REFERENCE mode.
+    cellshift (integrator.f90:256-389) rebuilds a cell's list as: its walls
+    (btot first), the fluid that stayed (in list order), then the arrivals
+    gathered from the cell below (tn of i+1), above (ts of i-1), right
+    (tw of j+1), left (te of j-1), upper left (tse), lower left (tne),
+    upper right (tsw), lower right (tnw), each in its source list's order;
+    resetid (isph.f90:96-109) numbers cells column by column, each column
+    from the top.  Two fluid particles arrive from each of the 8 neighbour
+    cells of a centre cell (one source lists them in reverse), two stay,
+    one wall sits there, and one particle sits still in each of the cells
+    above, below, left and right of it."""
+    h = 1.3 * DX
+    cs = 4.0 * h
+    prm = lattice_params(cell_order=ORDER_REFERENCE, cell_size=cs, xmin=0.0, ymin=0.0,
+                         xmax=4.0 * cs, ymax=4.0 * cs)
+    c0 = 1.5 * cs  # a point inside the centre cell [cs, 2cs)^2
+
+    def at(dx, dy, fx, fy):  # a point in the cell (dx, dy) from the centre
+        return (c0 + dx * cs + fx * cs, c0 + dy * cs + fy * cs)
+
+    names, start, end, tid = [], [], [], []
+
+    def add(name, p0, p1, t=3):
+        names.append(name)
+        start.append(p0)
+        end.append(p1)
+        tid.append(t)
+
+    add("stay_b", at(0, 0, 0.2, 0.1), at(0, 0, 0.25, 0.15))
+    add("wall", at(0, 0, -0.3, -0.3), at(0, 0, -0.3, -0.3), t=1)
+    add("stay_a", at(0, 0, -0.2, 0.2), at(0, 0, -0.1, 0.3))
+    gather = [("below", 0, -1), ("above", 0, 1), ("right", 1, 0), ("left", -1, 0),
+              ("upper_left", -1, 1), ("lower_left", -1, -1), ("upper_right", 1, 1),
+              ("lower_right", 1, -1)]
+    for name, dx, dy in gather:
+        # listed in reverse for the left source: its list order is b then a
+        pair = ("b", "a") if name == "left" else ("a", "b")
+        for k, tag in enumerate(pair):
+            off = -0.35 + 0.1 * k
+            add(f"{name}_{tag}", at(dx, dy, -0.45 * dx + off * (dx == 0), -0.45 * dy + off * (dy == 0)),
+                at(0, 0, 0.4 * dx + off * (dx == 0) + 0.02 * k * (dx != 0),
+                   0.4 * dy + off * (dy == 0) + 0.02 * k * (dy != 0)))
+    for name, dx, dy in [("still_up", 0, 1), ("still_down", 0, -1), ("still_left", -1, 0),
+                         ("still_right", 1, 0)]:
+        p = at(dx, dy, 0.3, 0.3)
+        add(name, p, p)
+    n = len(names)
+    parts = lattice(1)
+    parts = {k: np.repeat(v, n) for k, v in parts.items()}
+    parts["x"] = np.array([s[0] for s in start])
+    parts["y"] = np.array([s[1] for s in start])
+    parts["tid"] = np.array(tid, np.int32)
+    parts["pid"] = np.arange(n, dtype=np.int32)
+    sim = sim_from(parts, prm)
+    pid = sim.get_particles()["pid"]
+    sim.set_field("x", np.array([end[i][0] for i in pid]))
+    sim.set_field("y", np.array([end[i][1] for i in pid]))
+    sim.cellshift()
+    got = [names[i] for i in sim.get_particles()["pid"]]
+    centre = ["wall", "stay_b", "stay_a"]
+    for name, _, _ in gather:
+        centre += [f"{name}_b", f"{name}_a"] if name == "left" else [f"{name}_a", f"{name}_b"]
+    want = ["still_left", "still_up"] + centre + ["still_down", "still_right"]
+    assert got == want
 
 
 def jittered_block(nside, seed, amp=0.15):
