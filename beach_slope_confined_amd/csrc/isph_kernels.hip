@@ -798,10 +798,19 @@ __global__ void k_advance_time(SimScalars* ss) {
     if (threadIdx.x == 0 && blockIdx.x == 0) ss->t = ss->t + ss->dt;
 }
 
-// plain streaming copy (bandwidth and counter calibration)
+// plain streaming copy (bandwidth and counter calibration): four 16-byte
+// loads in flight per lane before their stores, then a one-deep tail
 __global__ void k_copy(const double2* __restrict__ a, double2* __restrict__ b, size_t n) {
     size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * BLOCK;
+    const size_t stride = (size_t)gridDim.x * BLOCK;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const double2 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride],
+                      v3 = a[i + 3 * stride];
+        b[i] = v0;
+        b[i + stride] = v1;
+        b[i + 2 * stride] = v2;
+        b[i + 3 * stride] = v3;
+    }
     for (; i < n; i += stride) b[i] = a[i];
 }
 

@@ -495,6 +495,14 @@ def main():
                                % (r64["nnz_off"] / r64["n"]),
                 "own_bytes_per_launch": own,
                 "own_frac": round(own / avg_s / 1e9 / HBM_PEAK_GBS, 4) if launches else None}
+    # attainable streaming bandwidth on this GPU: a 1 GiB copy (read + write)
+    try:
+        cms = r64["sim"].copy_bench(1 << 30, 10)
+        copy_gbs = 2 * (1 << 30) / cms / 1e6
+        roofline["copy_GBs"] = round(copy_gbs, 1)
+        roofline["frac_of_copy"] = round(achieved / copy_gbs, 4)
+    except Exception as e:  # reported, not hidden
+        roofline["copy_GBs"] = {"error": str(e)}
     sb = model_step_bytes(r64["n"], r64["nnz_off"], timed_iters[-1])
     step_model = {"bytes_per_step": round(sb), "achieved": round(sb / (ms / 1e3) / 1e9, 1),
                   "unit": "GB/s", "frac": round(sb / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
