@@ -271,18 +271,31 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __
         float* dst = As + 9 * B.off[l];
         for (int e = c; e < n9; e += MG_CMAX) dst[e] = src[e];
     }
+    // LDS keeps what the last kernel on this CU left there: every iterate
+    // starts from zero, and a sweep reads the coarser level's iterate only
+    // when it applies its correction
     xt[0][c] = 0.0f;
+    for (int e = c; e < MG_CLOW; e += MG_CMAX) {
+        xl[0][e] = 0.0f;
+        xl[1][e] = 0.0f;
+    }
     __syncthreads();
 
     // top sweep from parity p (z = x + e(parent) on level 1 at parity q if corr)
     auto top_sweep = [&](int p, bool corr, int q) {
         if (own) {
             const float* xc = xt[p];
-            const float* e = xl[q] + (K > 1 ? B.off[1] : 0);
             float acc = 0.0f;
-            const float f = corr ? B.over : 0.0f;
-            for (int d = 0; d < 9; ++d) acc += a[d] * (xc[nb[d]] + f * e[pa[d]]);
-            const float zi = xc[c] + f * e[pc];
+            float zi;
+            if (corr) {
+                const float* e = xl[q] + B.off[1];
+                const float f = B.over;
+                for (int d = 0; d < 9; ++d) acc += a[d] * (xc[nb[d]] + f * e[pa[d]]);
+                zi = xc[c] + f * e[pc];
+            } else {
+                for (int d = 0; d < 9; ++d) acc += a[d] * xc[nb[d]];
+                zi = xc[c];
+            }
             xt[p ^ 1][c] = live ? zi + w * (bc - acc) : 0.0f;
         }
         __syncthreads();
@@ -301,16 +314,28 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __
                 } else {
                     const int cx = c / ny, cy = c - cx * ny;
                     const float* xc = xl[p] + o;
-                    const int ny2 = corr ? B.ny[l + 1] : 0;
-                    const float* e = corr ? xl[q] + B.off[l + 1] : xc;
-                    const float f = corr ? B.over : 0.0f;
                     float acc = 0.0f;
+                    float zi;
+                    if (corr) {
+                        const int ny2 = B.ny[l + 1];
+                        const float* e = xl[q] + B.off[l + 1];
+                        const float f = B.over;
 #pragma unroll
-                    for (int d = 0; d < 9; ++d) {
-                        const int ex = mg_clamp(cx + d / 3 - 1, nx), ey = mg_clamp(cy + d % 3 - 1, ny);
-                        acc += A[d * n + c] * (xc[ex * ny + ey] + f * e[(ex >> 1) * ny2 + (ey >> 1)]);
+                        for (int d = 0; d < 9; ++d) {
+                            const int ex = mg_clamp(cx + d / 3 - 1, nx);
+                            const int ey = mg_clamp(cy + d % 3 - 1, ny);
+                            acc += A[d * n + c] * (xc[ex * ny + ey] + f * e[(ex >> 1) * ny2 + (ey >> 1)]);
+                        }
+                        zi = xc[c] + f * e[(cx >> 1) * ny2 + (cy >> 1)];
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < 9; ++d) {
+                            const int ex = mg_clamp(cx + d / 3 - 1, nx);
+                            const int ey = mg_clamp(cy + d % 3 - 1, ny);
+                            acc += A[d * n + c] * xc[ex * ny + ey];
+                        }
+                        zi = xc[c];
                     }
-                    const float zi = xc[c] + f * e[(cx >> 1) * ny2 + (cy >> 1)];
                     nv = zi + MG_OMEGA * (bl[o + c] - acc) / dg;
                 }
             }

@@ -91,3 +91,27 @@ def test_multigrid_steps_track_a_tight_fp64_run(require_gpu):
         print(f"step {step}: FP64 1e-8 {err['fp64']}, multigrid 1e-9 {err['mg']}")
         for k in range(2):
             assert err["mg"][k] <= max(2.0 * err["fp64"][k], 1e-7)
+
+
+def test_multigrid_solve_after_a_failed_one_in_another_context(require_gpu):
+    """The in-block coarse solve keeps its iterates in LDS, which holds what
+    the last kernel on the CU left there.  A solve whose right-hand side is
+    NaN fails (ISPH_E_NOCONV) and leaves NaN behind; a solve in a context
+    created afterwards must still converge exactly as one created before."""
+    cfg = CASES["dam_break"]()
+    before = _to_solve(cfg, PREC_MIXED_MG, 1e-9)
+    it0, rel0 = before.ppe_solve()
+    x0 = before.get_field("pressure")
+    before.close()
+    bad = _to_solve(cfg, PREC_MIXED_MG, 1e-9)
+    bad.set_field("rhs", np.full(bad.count(), np.nan))
+    with pytest.raises(capi.IsphError, match="NOCONV"):
+        bad.ppe_solve()
+    bad.close()
+    after = _to_solve(cfg, PREC_MIXED_MG, 1e-9)
+    it1, rel1 = after.ppe_solve()
+    x1 = after.get_field("pressure")
+    after.close()
+    print(f"before: {it0} iterations, {rel0:.3e}; after: {it1}, {rel1:.3e}", flush=True)
+    assert (it1, rel1) == (it0, rel0)
+    assert np.array_equal(x1, x0)
