@@ -127,7 +127,10 @@ struct isph_ctx {
 namespace isph {
 namespace host {
 
-inline int nblk(int64_t n, int b = BLOCK) { return (int)((n + b - 1) / b); }
+// blocks for n items, at least one: with no particles every per-particle
+// kernel still launches (its threads skip i >= n) and every reduction sees
+// one zero partial, so the stages keep their scalar bookkeeping
+inline int nblk(int64_t n, int b = BLOCK) { return n > 0 ? (int)((n + b - 1) / b) : 1; }
 inline int nblk8(int64_t n) { return ((nblk(n) + 7) / 8) * 8; }
 
 int fail(isph_ctx* c, int code, const char* m);
