@@ -95,6 +95,7 @@ void free_state(StateView& s) {
                  s.varts, s.velm, s.con, s.ovol, s.tid, s.flags, s.pid};
     for (auto q : p)
         if (q) (void)hipFree(q);
+    s = StateView{};
 }
 
 // read and clear the device error word; map to a status
@@ -1194,10 +1195,21 @@ int isph_checkpoint(isph_ctx* c, int32_t slot) {
     if (!c || slot < 0 || slot >= 64) return ISPH_E_ARG;
     auto& sl = c->slots[slot];
     if (!sl.v.pos) {
-        if (alloc_state(c, sl.v, (size_t)c->cap)) return fail(c, ISPH_E_HIP, "checkpoint allocation failed");
-        HIPCHK(c, hipMalloc((void**)&sl.key, sizeof(int) * c->cap));
-        HIPCHK(c, hipMalloc((void**)&sl.fsol, sizeof(double) * c->cap));
-        HIPCHK(c, hipMalloc((void**)&sl.bins, sizeof(int) * (c->grid.nbins + 2)));
+        // all or nothing: a slot is either complete or left empty
+        int rc = alloc_state(c, sl.v, (size_t)c->cap);
+        rc |= dalloc(c, &sl.key, (size_t)c->cap);
+        rc |= dalloc(c, &sl.fsol, (size_t)c->cap);
+        rc |= dalloc(c, &sl.bins, (size_t)c->grid.nbins + 2);
+        if (rc) {
+            free_state(sl.v);
+            void* q[] = {sl.key, sl.fsol, sl.bins};
+            for (auto x : q)
+                if (x) (void)hipFree(x);
+            sl.key = nullptr;
+            sl.fsol = nullptr;
+            sl.bins = nullptr;
+            return fail(c, ISPH_E_HIP, "checkpoint allocation failed");
+        }
     }
     int rc = copy_state(c, c->st[c->cur], sl.v, c->n);
     if (rc) return rc;
