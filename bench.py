@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--reference-run-only", action="store_true")
     ap.add_argument("--adjoint-dx", type=float, default=0.002)
     ap.add_argument("--adjoint-steps", type=int, default=50)
-    ap.add_argument("--cpu-iters", type=int, default=400,
+    ap.add_argument("--cpu-iters", type=int, default=2000,
                     help="BiCGSTAB iterations timed on the CPU oracle")
     ap.add_argument("--pmc-run", action="store_true",
                     help="short run for hardware-counter profiles: one step whose FP64 solve is "
