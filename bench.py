@@ -324,7 +324,7 @@ def run_reference_scenario(steps, cpu_steps):
     return out
 
 
-def run_adjoint(dx, nsteps, every, multigrid=False):
+def run_adjoint(dx, nsteps, every, multigrid=False, dt=None):
     """Adjoint/forward cost on the porous beach-slope dam break: nsteps FP64
     steps with checkpoints every `every` steps, objective = time-mean
     Shepard pressure at 3 probes on the slope face + water depth at
@@ -337,6 +337,8 @@ def run_adjoint(dx, nsteps, every, multigrid=False):
     cfg = configs.dam_break(dx)
     if multigrid:
         cfg["params"] = cfg["params"].copy(precision=PREC_MIXED_MG, tol=1e-9)
+    if dt is not None:
+        cfg["params"] = cfg["params"].copy(dt=dt)
     sim = configs.make_sim(cfg, capi.IsphSim)
     sim.checkpoint(0)
     obj = adjoint.slope_objective(gauges=(1.2, 0.3))
@@ -362,8 +364,9 @@ def run_adjoint(dx, nsteps, every, multigrid=False):
     sim.close()
     kind = "multigrid" if multigrid else "FP64"
     return {"workload": f"dam_break dx={dx:g}: {cfg['n_fluid']} fluid, {n} particles, "
-                        f"{len(cfg['soil']['x'])} soil; {nsteps} {kind} steps, checkpoints "
-                        f"every {every}",
+                        f"{len(cfg['soil']['x'])} soil; {nsteps} {kind} steps of "
+                        f"dt={cfg['params'].dt:g} with shifting, checkpoints every {every}",
+            "particle_steps_per_s": round(cfg["n_fluid"] * nsteps / plain, 1),
             "forward_s": round(fwd, 4), "plain_forward_s": round(plain, 4),
             "backward_s": round(bwd, 4), "adjoint_over_forward": round(bwd / fwd, 3),
             "J": J, "grad_por_alpha_beta": [float(v) for v in g],
@@ -682,6 +685,13 @@ def main():
             line["adjoint"]["multigrid"] = mg
         except Exception as e:  # reported, not hidden
             line["adjoint"] = {"error": str(e)}
+        # 500 steps with shifting and the multigrid PPE, and the adjoint of
+        # the whole run, on the finest dam break that stays stable for that
+        # long (dx = 0.001 at dt = 2.5e-5; the 4M beach does not)
+        try:
+            line["long_run"] = run_adjoint(0.001, 500, 10, multigrid=True, dt=2.5e-5)
+        except Exception as e:  # reported, not hidden
+            line["long_run"] = {"error": str(e)}
 
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, int(np.mean(timed_iters)))
