@@ -75,6 +75,10 @@ struct AdjointState {
     double* rsumT = nullptr;  // multigrid-preconditioned transposed solve
     double* dsc = nullptr;   // [0] aumax, [1] J, [2..25] objective sums, [26..29] gradients
     double* part = nullptr;  // 3 * OBJ_MAXP * nblk(cap)
+    // lambda of the step reversed last, by particle id: the warm start of
+    // the next transposed solve (the adjacent step, rows in another order)
+    double* lam_pid = nullptr;
+    int npid = 0;
 };
 
 namespace {
@@ -398,7 +402,9 @@ int adj_ppe(isph_ctx* c, const AdjView& a, const StateView& s, int* nbr, int* cn
                                                   nbr, cnt, L, c->val64, nullptr, nullptr, c->dpos,
                                                   c->rhs, ss, c->err);
     k_transpose_vals<<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, nbr, cnt, c->val64, S->valT, c->err);
-    HIPCHK(c, hipMemsetAsync(S->lam, 0, sizeof(double) * n, c->stream));
+    // warm start: each particle's lambda of the step reversed before this
+    // one (zero for a particle that step did not have)
+    k_lam_gather<<<nblk(n), BLOCK, 0, c->stream>>>(n, s.pid, S->lam_pid, S->npid, S->lam);
     int rc = take_errors(c);
     if (!rc && pr.precision == ISPH_PREC_MIXED_MG) {
         // the forward solves with the multigrid preconditioner; so does the
@@ -431,6 +437,7 @@ int adj_ppe(isph_ctx* c, const AdjView& a, const StateView& s, int* nbr, int* cn
     }
     if (!rc) rc = read_sc(c);
     if (!rc && iters) *iters += c->sc_host->total;
+    if (!rc) k_lam_scatter<<<nblk(n), BLOCK, 0, c->stream>>>(n, s.pid, S->lam, S->lam_pid, S->npid);
     c->nbr = nbr0;
     c->cnt = cnt0;
     c->assembled = false;
@@ -542,6 +549,7 @@ void adj_free(isph_ctx* c) {
         for (auto q : p)
             if (q) (void)hipFree(q);
     }
+    if (S->lam_pid) (void)hipFree(S->lam_pid);
     delete S;
     c->adj = nullptr;
 }
@@ -597,6 +605,16 @@ int isph_adjoint_backward(isph_ctx* c, double* grad3, double* J, int64_t* adj_it
         return fail(c, ISPH_E_STATE, "parameters changed since isph_adjoint_forward");
     int rc;
     if ((rc = zero_all(c))) return rc;
+    // the transposed solves' warm starts by particle id, zero at the start
+    const int npid = std::max(c->next_pid, 1);
+    if (S->npid < npid) {
+        if (S->lam_pid) (void)hipFree(S->lam_pid);
+        S->lam_pid = nullptr;
+        S->npid = 0;
+        if (dalloc(c, &S->lam_pid, (size_t)npid)) return fail(c, ISPH_E_HIP, "adjoint buffer allocation failed");
+        S->npid = npid;
+    }
+    k_zero<<<nblk(S->npid), BLOCK, 0, c->stream>>>(S->npid, S->lam_pid);
     const int every = S->every, nsteps = S->nsteps;
     const int nseg = (nsteps + every - 1) / every;
     const double weight = 1.0 / nsteps;

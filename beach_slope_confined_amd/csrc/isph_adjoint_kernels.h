@@ -1333,5 +1333,23 @@ __global__ void k_zero(int64_t count, double* __restrict__ x) {
     if (i < count) x[i] = 0.0;
 }
 
+// lambda by particle id <-> lambda by row (ids are unique; an id outside
+// [0, npid) has no stored value and starts from zero)
+__global__ void k_lam_gather(int n, const int* __restrict__ pid, const double* __restrict__ by_pid,
+                             int npid, double* __restrict__ lam) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int q = pid[i];
+    lam[i] = (q >= 0 && q < npid) ? by_pid[q] : 0.0;
+}
+
+__global__ void k_lam_scatter(int n, const int* __restrict__ pid, const double* __restrict__ lam,
+                              double* __restrict__ by_pid, int npid) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int q = pid[i];
+    if (q >= 0 && q < npid) by_pid[q] = lam[i];
+}
+
 }  // namespace adj
 }  // namespace isph
