@@ -78,7 +78,9 @@ struct AdjointState {
     // lambda of the step reversed last, by particle id: the warm start of
     // the next transposed solve (the adjacent step, rows in another order)
     double* lam_pid = nullptr;
-    int npid = 0;
+    int npid = 0;       // entries allocated in lam_pid
+    int warm_npid = 0;  // entries the transposed solves read and write: npid in
+                        // the backward sweep, 0 (zero start) otherwise
 };
 
 namespace {
@@ -404,7 +406,7 @@ int adj_ppe(isph_ctx* c, const AdjView& a, const StateView& s, int* nbr, int* cn
     k_transpose_vals<<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, nbr, cnt, c->val64, S->valT, c->err);
     // warm start: each particle's lambda of the step reversed before this
     // one (zero for a particle that step did not have)
-    k_lam_gather<<<nblk(n), BLOCK, 0, c->stream>>>(n, s.pid, S->lam_pid, S->npid, S->lam);
+    k_lam_gather<<<nblk(n), BLOCK, 0, c->stream>>>(n, s.pid, S->lam_pid, S->warm_npid, S->lam);
     int rc = take_errors(c);
     if (!rc && pr.precision == ISPH_PREC_MIXED_MG) {
         // the forward solves with the multigrid preconditioner; so does the
@@ -437,7 +439,8 @@ int adj_ppe(isph_ctx* c, const AdjView& a, const StateView& s, int* nbr, int* cn
     }
     if (!rc) rc = read_sc(c);
     if (!rc && iters) *iters += c->sc_host->total;
-    if (!rc) k_lam_scatter<<<nblk(n), BLOCK, 0, c->stream>>>(n, s.pid, S->lam, S->lam_pid, S->npid);
+    if (!rc && S->warm_npid > 0)
+        k_lam_scatter<<<nblk(n), BLOCK, 0, c->stream>>>(n, s.pid, S->lam, S->lam_pid, S->warm_npid);
     c->nbr = nbr0;
     c->cnt = cnt0;
     c->assembled = false;
@@ -531,6 +534,9 @@ int check_adjoint_params(isph_ctx* c) {
         return fail(c, ISPH_E_ARG, "the adjoint runs with cell_order = ISPH_ORDER_CELL");
     if (c->prm.precision == ISPH_PREC_FGMRES)
         return fail(c, ISPH_E_ARG, "the adjoint needs a BiCGSTAB forward solve (not FGMRES)");
+    // the backward's recomputation does not replay buffer removal/insertion
+    if (c->nzones > 0)
+        return fail(c, ISPH_E_ARG, "the adjoint does not support buffer zones (none may be installed)");
     return ISPH_OK;
 }
 
@@ -603,18 +609,27 @@ int isph_adjoint_backward(isph_ctx* c, double* grad3, double* J, int64_t* adj_it
     if (!S || !S->ready) return fail(c, ISPH_E_STATE, "isph_adjoint_backward needs isph_adjoint_forward first");
     if (memcmp(&S->prm, &c->prm, sizeof(isph_params)) != 0)
         return fail(c, ISPH_E_STATE, "parameters changed since isph_adjoint_forward");
+    if (c->nzones > 0)
+        return fail(c, ISPH_E_ARG, "the adjoint does not support buffer zones (none may be installed)");
     int rc;
     if ((rc = zero_all(c))) return rc;
-    // the transposed solves' warm starts by particle id, zero at the start
-    const int npid = std::max(c->next_pid, 1);
-    if (S->npid < npid) {
-        if (S->lam_pid) (void)hipFree(S->lam_pid);
-        S->lam_pid = nullptr;
-        S->npid = 0;
-        if (dalloc(c, &S->lam_pid, (size_t)npid)) return fail(c, ISPH_E_HIP, "adjoint buffer allocation failed");
-        S->npid = npid;
+    // the transposed solves' warm starts by particle id, zero at the start.
+    // A sparse id space (max pid far above the particle count) would need a
+    // table far larger than the state: those solves start from zero instead.
+    const int64_t npid = std::max<int64_t>(c->next_pid, 1);
+    S->warm_npid = 0;
+    if (npid <= 4 * (int64_t)c->cap + 1024) {
+        if (S->npid < npid) {
+            if (S->lam_pid) (void)hipFree(S->lam_pid);
+            S->lam_pid = nullptr;
+            S->npid = 0;
+            if (dalloc(c, &S->lam_pid, (size_t)npid))
+                return fail(c, ISPH_E_HIP, "adjoint buffer allocation failed");
+            S->npid = (int)npid;
+        }
+        k_zero<<<nblk(npid), BLOCK, 0, c->stream>>>((int)npid, S->lam_pid);
+        S->warm_npid = (int)npid;
     }
-    k_zero<<<nblk(S->npid), BLOCK, 0, c->stream>>>(S->npid, S->lam_pid);
     const int every = S->every, nsteps = S->nsteps;
     const int nseg = (nsteps + every - 1) / every;
     const double weight = 1.0 / nsteps;
@@ -782,8 +797,11 @@ int isph_adjoint_stage(isph_ctx* c, int32_t stage, const isph_objective* obj) {
         case ISPH_ADJ_VARTS: rc = adj_strain(c, a, s, c->nbr, c->cnt, c->L, c->ss, 0); break;
         case ISPH_ADJ_VSTAR: rc = adj_vstar(c, a, s, c->nbr, c->cnt, c->L, c->ss); break;
         case ISPH_ADJ_PPESOLVE: {
-            // p: the pressure of the forward solve (the adjoint leaves s.p alone)
+            // p: the pressure of the forward solve (the adjoint leaves s.p alone).
+            // The transposed solve starts from zero: the by-pid warm start
+            // belongs to the backward sweep, not to a lone stage.
             int64_t it = 0;
+            S->warm_npid = 0;
             rc = adj_ppe(c, a, s, c->nbr, c->cnt, c->L, s.p, c->ss, &it);
             break;
         }

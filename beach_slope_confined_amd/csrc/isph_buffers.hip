@@ -131,9 +131,11 @@ static int insert_buffer(isph_ctx* c, int zi) {
     if (z.nins == 0) return ISPH_OK;
     if ((int64_t)c->n + z.nins > c->cap)
         return fail(c, ISPH_E_CAPACITY, "buffer insertion exceeds the particle capacity");
+    if (c->next_pid + z.nins > (int64_t)INT32_MAX + 1)
+        return fail(c, ISPH_E_CAPACITY, "particle ids exhausted: inserted pids would exceed INT32_MAX");
     k_buffer_insert<<<nblk(z.nins), BLOCK, 0, c->stream>>>(
         z.nins, c->n, z.pos, z.vel, z.p, z.flags, z.key, z.mass, z.density, z.ovol, z.con,
-        c->next_pid, c->st[c->cur], c->key_s);
+        (int)c->next_pid, c->st[c->cur], c->key_s);
     HIPCHK(c, hipGetLastError());
     c->n += z.nins;
     c->next_pid += z.nins;

@@ -879,7 +879,7 @@ int isph_set_particles(isph_ctx* c, int32_t n, const double* x, const double* y,
     HIPCHK(c, hipMemcpyAsync(c->ss, &ss0, sizeof(ss0), H2D, c->stream));
     c->n = n;
     c->next_pid = 0;
-    for (int i = 0; i < n; ++i) c->next_pid = std::max(c->next_pid, pid[i] + 1);
+    for (int i = 0; i < n; ++i) c->next_pid = std::max(c->next_pid, (int64_t)pid[i] + 1);
     int rc = sort_particles(c, false, true);
     if (rc) return rc;
     // fsol = fguess: the uploaded pressures in the first row order
@@ -1221,6 +1221,10 @@ int isph_checkpoint(isph_ctx* c, int32_t slot) {
     HIPCHK(c, hipMemcpyAsync(&sl.ss, c->ss, sizeof(SimScalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     sl.n = c->n;
+    sl.nzones = c->nzones;
+    for (int k = 0; k < c->nzones; ++k) sl.zone_counter[k] = c->zones[k].counter;
+    sl.zone_shift = c->zone_shift;
+    sl.next_pid = c->next_pid;
     sl.used = true;
     return ISPH_OK;
 }
@@ -1229,6 +1233,8 @@ int isph_restore(isph_ctx* c, int32_t slot) {
     if (!c || slot < 0 || slot >= 64) return ISPH_E_ARG;
     auto& sl = c->slots[slot];
     if (!sl.used) return fail(c, ISPH_E_STATE, "restore from an empty checkpoint slot");
+    if (sl.nzones != c->nzones)
+        return fail(c, ISPH_E_STATE, "buffer zones were installed or removed since this checkpoint");
     int rc = copy_state(c, sl.v, c->st[c->cur], sl.n);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(c->key_s, sl.key, sizeof(int) * sl.n, hipMemcpyDeviceToDevice, c->stream));
@@ -1239,6 +1245,9 @@ int isph_restore(isph_ctx* c, int32_t slot) {
     HIPCHK(c, hipMemcpyAsync(c->ss, &sl.ss, sizeof(SimScalars), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->n = sl.n;
+    for (int k = 0; k < c->nzones; ++k) c->zones[k].counter = sl.zone_counter[k];
+    c->zone_shift = sl.zone_shift;
+    c->next_pid = sl.next_pid;
     c->have_nbr = false;
     c->assembled = false;
     return ISPH_OK;

@@ -101,6 +101,13 @@ struct isph_ctx {
         int* bins = nullptr;  // bin starts of the last sort (the neighbour search reads them)
         int n = 0;
         bool used = false;
+        // the buffer schedule: each zone's next event, the time shift and
+        // the next inserted pid, so that steps after a restore insert and
+        // remove as a fresh run from the same state does
+        long long zone_counter[ISPH_MAX_ZONES] = {};
+        int nzones = 0;
+        double zone_shift = 0.0;
+        int64_t next_pid = 0;
     } slots[64];
     isph::host::AdjointState* adj = nullptr;
     isph::host::MGData* mg = nullptr;
@@ -111,7 +118,7 @@ struct isph_ctx {
     isph::host::BufferZone zones[ISPH_MAX_ZONES];
     int nzones = 0;
     double zone_shift = 0.0;
-    int next_pid = 0;              // pid of the next inserted particle
+    int64_t next_pid = 0;          // pid of the next inserted particle (int64: max pid + 1 may exceed int)
     std::string msg;
 };
 

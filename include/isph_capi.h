@@ -22,7 +22,12 @@
  *     current order.  After isph_cellshift()/isph_cellshiftalt() that order
  *     is the cell-sorted order, which is also the PPE row/column order (the
  *     reference's `matid`, `isph.f90:88-113`).  `pid` travels with the
- *     particle and identifies it across re-orderings.
+ *     particle and identifies it across re-orderings.  pids must be unique
+ *     and non-negative (not checked): buffer insertion hands out max pid + 1
+ *     upward, and the adjoint's warm starts are kept by pid (with repeated
+ *     pids they, and so the gradient within solver tolerance, vary from run
+ *     to run).  A pid space much larger than the particle count costs the
+ *     adjoint its warm starts, not memory.
  *   - particle types follow the reference (`particle.f90:92`,
  *     `boundary.f90:50-83`): tid 1 = fixed wall (innermost layer),
  *     tid 2 = dummy wall, tid 3 = fluid.  Soil particles (the reference's
