@@ -266,7 +266,9 @@ int run_comp_vel(isph_ctx* c) {
 }
 
 int run_comp_pos(isph_ctx* c) {
-    if (c->n) k_comp_pos<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->st[c->cur], c->ss);
+    if (c->n)
+        k_comp_pos<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, c->st[c->cur], c->ss,
+                                                        c->prm.xrelease_lo, c->prm.xrelease_hi);
     HIPCHK(c, hipGetLastError());
     return ISPH_OK;
 }

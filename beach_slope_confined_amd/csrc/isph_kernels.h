@@ -321,7 +321,8 @@ __global__ void k_mg_lower(int n, size_t ld, const int* cnt, const int* dpos, co
                            float* val32, double* rsum);
 __global__ void k_comp_vel(int n, KernelConsts kc, size_t ld, StateView s, const int* nbr,
                            const int* cnt, const double* L, const SimScalars* ss, uint32_t* err);
-__global__ void k_comp_pos(int n, StateView s, const SimScalars* ss);
+__global__ void k_comp_pos(int n, StateView s, const SimScalars* ss, double xrel_lo,
+                           double xrel_hi);
 __global__ void k_freesurf(int n, double thr, size_t ld, StateView s, const int* nbr,
                            const int* cnt, const double* divr);
 __global__ void k_shift(int n, KernelConsts kc, double csh, double cap, size_t ld, StateView s,

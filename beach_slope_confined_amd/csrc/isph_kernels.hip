@@ -630,16 +630,23 @@ __global__ void __launch_bounds__(BLOCK) k_comp_vel(int n, KernelConsts kc, size
 }
 
 // comp_pos, integrator.f90:867-906 (buffer release :888-897 not modelled)
-__global__ void k_comp_pos(int n, StateView s, const SimScalars* ss) {
+// comp_pos, integrator.f90:867-906: the midpoint position update, then the
+// release of a buffer particle that has moved into (xrel_lo, xrel_hi)
+// (:888-897): it stops being a buffer particle.
+__global__ void k_comp_pos(int n, StateView s, const SimScalars* ss, double xrel_lo,
+                           double xrel_hi) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    if (s.tid[i] == 3 && !(s.flags[i] & ISPH_FLAG_DEAD)) {
+    const int f = s.flags[i];
+    if (s.tid[i] == 3 && !(f & ISPH_FLAG_DEAD)) {
         const double dt = ss->dt;
         double2 p = s.pos[i], q = s.sp[i], v = s.vel[i];
         const double nn = s.por[i];
         p.x = ((p.x + q.x) + (dt * v.x) / nn) / 2.0;
         p.y = ((p.y + q.y) + (dt * v.y) / nn) / 2.0;
         s.pos[i] = p;
+        if ((f & ISPH_FLAG_BUFFER) && p.x > xrel_lo && p.x < xrel_hi)
+            s.flags[i] = f & ~ISPH_FLAG_BUFFER;
     }
 }
 

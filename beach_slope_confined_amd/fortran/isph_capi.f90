@@ -22,6 +22,7 @@ module isph_capi
         real(c_double) :: h, cell_size, xmin, ymin, xmax, ymax, dl, lam, dt, rho0, rel_den, g, nu
         real(c_double) :: csh, maxshift, lamfs, maxdivr, por, alpha, beta, d50, tol, inner_tol
         real(c_double) :: xlcutoff, ytcutoff, sig1, mu, rhomin, tschmidt, salt_xbuf
+        real(c_double) :: xrelease_lo, xrelease_hi
         integer(c_int32_t) :: kernel, drag_law, precision, maxit, inner_maxit, max_outer, max_nbr
         integer(c_int32_t) :: shifting, adaptive_dt, sink, visc_law, salinity, cell_order, reserved
     end type isph_params

@@ -44,7 +44,7 @@ _DOUBLES = [
     "h", "cell_size", "xmin", "ymin", "xmax", "ymax", "dl", "lam", "dt", "rho0",
     "rel_den", "g", "nu", "csh", "maxshift", "lamfs", "maxdivr", "por", "alpha",
     "beta", "d50", "tol", "inner_tol", "xlcutoff", "ytcutoff", "sig1", "mu",
-    "rhomin", "tschmidt", "salt_xbuf",
+    "rhomin", "tschmidt", "salt_xbuf", "xrelease_lo", "xrelease_hi",
 ]
 _INTS = [
     "kernel", "drag_law", "precision", "maxit", "inner_maxit", "max_outer",
@@ -86,7 +86,7 @@ def make_params(**kw):
         inner_maxit=0, max_outer=200, max_nbr=64, shifting=1, adaptive_dt=0,
         sink=0, xlcutoff=-1e300, ytcutoff=1e300, sig1=0.2, mu=1e-3, salinity=0,
         cell_order=ORDER_CELL, reserved=0,
-        rhomin=1000.0, tschmidt=1.0, salt_xbuf=1.6,
+        rhomin=1000.0, tschmidt=1.0, salt_xbuf=1.6, xrelease_lo=1e300, xrelease_hi=1e300,
     )
     defaults.update(kw)
     for k, v in defaults.items():

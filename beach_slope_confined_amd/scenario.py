@@ -100,6 +100,11 @@ def reference_setup(**over):
              rv_buf_r=(brrealx * (2 * bl - 1)) + (fpx - 1) * 2 * prrealx / sq + prrealx / sq
              + domain_shift + 3 * 2 * prrealx / sq + 0.5 * prrealx / sq,
              rv_buf_l=2 * prrealx + brrealx + 0.5 * prrealx)
+    # the buffer release window of comp_pos (integrator.f90:888-889), with
+    # xrcutoff from setup.f90:76 (fpx of :72, domain_shift of :56)
+    s.update(xrcutoff=(((brrealx * (2 * bl - 1)) + ((s["fpx"] - 1) * 2) * prrealx / sq)
+                       + (3 * prrealx) / sq + domain_shift) - 0.001,
+             xrelease_lo=((8 * prrealx) + brrealx) + 0.001)
     return s
 
 
@@ -275,7 +280,8 @@ def reference_scenario(**over):
                       adaptive_dt=1, cell_size=cs, xmin=-cs, ymin=-cs,
                       xmax=(s["cellx"] - 1) * cs - 1e-12, ymax=(s["celly"] - 1) * cs - 1e-12,
                       cell_order=ORDER_REFERENCE, sink=0, xlcutoff=s["xlcutoff"],
-                      salt_xbuf=1.6, rhomin=1000.0, tschmidt=1.0, max_nbr=96)
+                      salt_xbuf=1.6, rhomin=1000.0, tschmidt=1.0, max_nbr=96,
+                      xrelease_lo=s["xrelease_lo"], xrelease_hi=s["xrcutoff"])
 
     def zone(templ, entry, period, xcut, remove_right, con):
         # template rows are (x, y, tid, mass, density, oden, vx, vy, pressure, flags);

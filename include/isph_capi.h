@@ -138,6 +138,11 @@ typedef struct isph_params {
     double tschmidt;   /* turbulent Schmidt number (initialize.f90:23)         */
     double salt_xbuf;  /* buffer particles with x < salt_xbuf are reset to
                           con = 0.5 after each transport substep (scalar.f90:252) */
+    double xrelease_lo; /* comp_pos releases a buffer particle (it becomes fluid)
+                           once xrelease_lo < x < xrelease_hi (integrator.f90:888-897:
+                           8 prrealx + brrealx + 0.001 and xrcutoff, setup.f90:76);
+                           the defaults (both 1e300) release nothing */
+    double xrelease_hi;
     int32_t kernel;      /* ISPH_KERNEL_*                                       */
     int32_t drag_law;    /* ISPH_DRAG_*                                         */
     int32_t precision;   /* ISPH_PREC_*                                         */

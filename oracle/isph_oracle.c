@@ -1629,14 +1629,19 @@ int ora_comp_vel(ora_ctx* c) {
     return ISPH_OK;
 }
 
-/* integrator.f90:867-906 comp_pos (buffer release at :888-897 not modelled) */
+/* integrator.f90:867-906 comp_pos, with the buffer release of :888-897: a
+ * buffer particle with xrelease_lo < x < xrelease_hi after the update stops
+ * being a buffer particle */
 int ora_comp_pos(ora_ctx* c) {
     const double dt = c->dt;
+    const double lo = c->prm.xrelease_lo, hi = c->prm.xrelease_hi;
 #pragma omp parallel for schedule(static)
     for (int32_t i = 0; i < c->n; ++i) {
         if (c->tid[i] == 3 && !(c->flags[i] & ISPH_FLAG_DEAD)) {
             c->x[i] = ((c->x[i] + c->xs[i]) + (dt * c->vx[i]) / c->por[i]) / 2.0;
             c->y[i] = ((c->y[i] + c->ys[i]) + (dt * c->vy[i]) / c->por[i]) / 2.0;
+            if ((c->flags[i] & ISPH_FLAG_BUFFER) && c->x[i] > lo && c->x[i] < hi)
+                c->flags[i] &= ~ISPH_FLAG_BUFFER;
         }
     }
     return ISPH_OK;

@@ -249,3 +249,34 @@ def test_drag_update_is_stationary_at_the_ergun_terminal_velocity(por):
         deltas.append(vys[0] + U)
         assert abs(deltas[-1]) < 0.01 * abs(g) * por * dt
     assert 3.8 < deltas[0] / deltas[1] < 4.2
+
+
+# ------------------------------------------------------- buffer release ---
+def test_comp_pos_releases_buffer_particles_inside_the_window():
+    """integrator.f90:888-897: after the position update a live buffer fluid
+    particle with xrelease_lo < x < xrelease_hi (strict) stops being a buffer
+    particle; one outside the window, on its edge, dead, or not fluid keeps
+    its flags.  Positions are set so that the update leaves them in place
+    (xs = x, v = 0): x_new = (x + xs + dt v/n)/2 = x exactly."""
+    from beach_slope_confined_amd.params import FLAG_BUFFER, FLAG_DEAD
+    lo, hi = 0.40625, 0.59375            # binary fractions: x == lo is exact
+    x = np.array([0.39, 0.41, 0.5, 0.59, 0.61, lo, hi, 0.5, 0.5, 0.5])
+    flags = np.full(x.size, FLAG_BUFFER, np.int32)
+    flags[7] = FLAG_BUFFER | FLAG_DEAD   # dead: comp_pos skips it
+    flags[8] = 0                         # plain fluid
+    tid = np.full(x.size, 3, np.int32)
+    tid[9] = 2                           # a wall particle with the flag
+    n = x.size
+    parts = dict(x=x, y=np.linspace(0.3, 0.7, n), vx=np.zeros(n), vy=np.zeros(n),
+                 mass=np.full(n, 1000.0 * DX * DX), density=np.full(n, 1000.0),
+                 oden=np.full(n, 1000.0), pressure=np.zeros(n), tid=tid, flags=flags,
+                 pid=np.arange(n, dtype=np.int32))
+    sim = sim_from(parts, lattice_params(xrelease_lo=lo, xrelease_hi=hi))
+    sim.set_field("xs", sim.get_field("x"))
+    sim.set_field("ys", sim.get_field("y"))
+    sim.comp_pos()
+    P = sim.get_particles()
+    order = np.argsort(P["pid"])
+    assert np.array_equal(P["x"][order], x)
+    buf = (P["flags"][order] & FLAG_BUFFER) != 0
+    assert buf.tolist() == [True, False, False, False, True, True, True, True, False, True]
