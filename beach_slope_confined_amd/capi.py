@@ -196,6 +196,8 @@ def load_library(path=LIB_PATH):
             ("snapshot_record", [ctypes.c_double] * 6 + [ctypes.c_char_p], ctypes.c_int),
             ("write_probe_series", [ctypes.c_char_p, ctypes.c_int32, _c_dp, _c_dp],
              ctypes.c_int),
+            ("probe_noalloc", [ctypes.c_void_p, ctypes.c_int32, _c_dp, _c_dp, _c_dp],
+             ctypes.c_int),
         ]:
             f = getattr(lib, "isph_" + name)
             f.argtypes = args

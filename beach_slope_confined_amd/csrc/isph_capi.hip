@@ -714,6 +714,12 @@ int isph_create(const isph_params* prm, int32_t cap, int32_t cap_soil, isph_ctx*
         delete c;
         return ISPH_E_HIP;
     }
+    // the device address of this GPU's isph_probe_noalloc scratch array
+    if ((rc = probe_scratch_attach(c))) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     const size_t N = (size_t)cap, M = (size_t)prm->max_nbr;
     const int64_t nb = c->grid.nbins;
     rc |= alloc_state(c, c->st[0], N);

@@ -53,6 +53,8 @@ struct BufferZone {
 }  // namespace host
 }  // namespace isph
 
+#define ISPH_MAX_DEVICES 64  // GPUs per process the library keeps per-device addresses for
+
 struct isph_ctx {
     isph_params prm;
     int32_t cap = 0, cap_soil = 0, n = 0, ns = 0;
@@ -119,6 +121,7 @@ struct isph_ctx {
     int nzones = 0;
     double zone_shift = 0.0;
     int64_t next_pid = 0;          // pid of the next inserted particle (int64: max pid + 1 may exceed int)
+    double* probe_scratch = nullptr;  // this GPU's isph_probe_noalloc scratch (device address)
     std::string msg;
 };
 
@@ -141,6 +144,7 @@ inline int nblk(int64_t n, int b = BLOCK) { return n > 0 ? (int)((n + b - 1) / b
 inline int nblk8(int64_t n) { return ((nblk(n) + 7) / 8) * 8; }
 
 int fail(isph_ctx* c, int code, const char* m);
+int probe_scratch_attach(isph_ctx* c);
 PorousParams porous_params(const isph_params& p);
 KernelConsts kconsts(const isph_ctx* c);
 

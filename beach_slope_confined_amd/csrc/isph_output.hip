@@ -18,11 +18,19 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "isph_host.h"
 #include "isph_kernels.h"
+
+// Device memory of isph_probe_noalloc: the probe abscissae, ordinates and
+// values of one call, 3 * ISPH_PROBE_NOALLOC_MAX doubles (96 KiB).  A
+// file-scope array, so the library never allocates it: each GPU holds its
+// own copy from load until the process ends.  In host code `scratch` only
+// names the symbol; its device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[3 * ISPH_PROBE_NOALLOC_MAX];
 
 namespace isph {
 
@@ -64,7 +72,38 @@ __global__ void k_probe(int np, const double* __restrict__ px, const double* __r
     out[k] = count ? pr / term1 : 0.0;
 }
 
+__global__ void k_probe_zero(int np, double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < np) out[k] = 0.0;
+}
+
 namespace host {
+
+namespace {
+std::mutex scratch_mu;
+double* scratch_dev[ISPH_MAX_DEVICES] = {};  // device address of `scratch`, per GPU
+}  // namespace
+
+// Takes the device address of this GPU's `scratch` array (once per device,
+// with that device current) and keeps it in the context.  Called by
+// isph_create on the context's device.
+int probe_scratch_attach(isph_ctx* c) {
+    int dev = -1;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev < 0 || dev >= ISPH_MAX_DEVICES) return fail(c, ISPH_E_HIP, "device index out of range");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            c->msg = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return ISPH_E_HIP;
+        }
+        scratch_dev[dev] = static_cast<double*>(p);
+    }
+    c->probe_scratch = scratch_dev[dev];
+    return ISPH_OK;
+}
 
 static void right_justify(const char* s, int len, int w, char* out) {
     if (len > w) {
@@ -237,6 +276,31 @@ int isph_probe(isph_ctx* c, int32_t np, const double* px, const double* py, doub
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(d);
     if (rc) return rc;
+    return sync_check(c);
+}
+
+int isph_probe_noalloc(isph_ctx* c, int32_t np, const double* px, const double* py, double* out) {
+    if (!c || np < 0 || (np && (!px || !py || !out))) return ISPH_E_ARG;
+    if (np > ISPH_PROBE_NOALLOC_MAX)
+        return fail(c, ISPH_E_ARG, "isph_probe_noalloc: np exceeds ISPH_PROBE_NOALLOC_MAX (4096)");
+    if (!c->probe_scratch)
+        return fail(c, ISPH_E_STATE, "isph_probe_noalloc: no scratch address for this context");
+    if (np == 0) return ISPH_OK;
+    // the GPU's scratch array (its device address, taken at isph_create):
+    // x at [0, np), y at [np, 2np), the values at [2np, 3np)
+    double* d = c->probe_scratch;
+    HIPCHK(c, hipMemcpyAsync(d, px, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d + np, py, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    if (c->n) {
+        k_probe<<<(np + 63) / 64, 64, 0, c->stream>>>(np, d, d + np, c->grid, kconsts(c),
+                                                     c->st[c->cur], c->bincnt, d + 2 * np, c->err);
+    } else {
+        // no particles: every value is 0 (written by a kernel, not a memset)
+        k_probe_zero<<<(np + 63) / 64, 64, 0, c->stream>>>(np, d + 2 * np);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(out, d + 2 * np, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return sync_check(c);
 }
 

@@ -339,6 +339,19 @@ int isph_combined(isph_ctx* ctx, const char* dir, int32_t index);
  * 2h in the 3x3 cells around the point's cell; 0 where no fluid is in range. */
 int isph_probe(isph_ctx* ctx, int32_t np, const double* px, const double* py,
                double* out);
+/* isph_probe without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving or sampling loops, harnesses that
+ * forbid hipMalloc).  Same arguments, results (bit for bit) and errors as
+ * isph_probe, except that np may not exceed ISPH_PROBE_NOALLOC_MAX
+ * (ISPH_E_ARG beyond it).  The points and values are staged in a fixed
+ * device array of 3 * ISPH_PROBE_NOALLOC_MAX doubles that every context on
+ * a GPU shares and that holds the last call's contents; calls that overlap
+ * in time on one GPU (from several threads or contexts) race.  Synchronous
+ * when it returns. */
+#define ISPH_PROBE_NOALLOC_MAX 4096
+int isph_probe_noalloc(isph_ctx* ctx, int32_t np, const double* px, const double* py,
+                       double* out);
 /* The probe series file of output.f90:150-166: n records (F10.3,1X,F10.3)
  * of a[i], b[i] (the reference writes t sqrt(|g|/wc) and
  * p / (rho |g| wc), probe.f90:126-127). */
