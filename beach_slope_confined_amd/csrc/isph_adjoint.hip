@@ -408,7 +408,7 @@ int adj_ppe(isph_ctx* c, const AdjView& a, const StateView& s, int* nbr, int* cn
     // one (zero for a particle that step did not have)
     k_lam_gather<<<nblk(n), BLOCK, 0, c->stream>>>(n, s.pid, S->lam_pid, S->warm_npid, S->lam);
     int rc = take_errors(c);
-    if (!rc && pr.precision == ISPH_PREC_MIXED_MG) {
+    if (!rc && prec_mg(pr.precision)) {
         // the forward solves with the multigrid preconditioner; so does the
         // transposed solve, on A^T's own Galerkin hierarchy
         if (!S->val32T) {

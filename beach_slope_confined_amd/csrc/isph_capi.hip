@@ -45,7 +45,12 @@ GridInfo make_grid(const isph_params& p) {
 
 bool prec_mixed(int precision) {
     return precision == ISPH_PREC_MIXED || precision == ISPH_PREC_MIXED_V64 ||
-           precision == ISPH_PREC_MIXED_MG;
+           precision == ISPH_PREC_MIXED_MG || precision == ISPH_PREC_IR ||
+           precision == ISPH_PREC_FP64_MG;
+}
+
+bool prec_mg(int precision) {
+    return precision == ISPH_PREC_MIXED_MG || precision == ISPH_PREC_FP64_MG;
 }
 
 int check_params(isph_ctx* c, const isph_params* p) {
@@ -54,7 +59,7 @@ int check_params(isph_ctx* c, const isph_params* p) {
         return fail(c, ISPH_E_ARG, "invalid geometry parameters");
     if (p->cell_order != ISPH_ORDER_CELL && p->cell_order != ISPH_ORDER_REFERENCE)
         return fail(c, ISPH_E_ARG, "unknown cell_order");
-    if (p->precision < ISPH_PREC_FP64 || p->precision > ISPH_PREC_FGMRES)
+    if (p->precision < ISPH_PREC_FP64 || p->precision > ISPH_PREC_FP64_MG)
         return fail(c, ISPH_E_ARG, "unknown precision");
     if (p->cell_order == ISPH_ORDER_REFERENCE && !(p->cell_size >= 4.0 * p->h))
         return fail(c, ISPH_E_ARG, "the reference order needs cell_size >= 4h (2x2-cell blocks)");
@@ -246,8 +251,7 @@ int run_assemble(isph_ctx* c) {
         k_assemble<<<nblk(c->n), BLOCK, 0, c->stream>>>(
             c->n, kconsts(c), p.lam, p.rho0 * p.rel_den, (size_t)c->cap, c->st[c->cur], c->nbr,
             c->cnt, c->L, c->val64, prec_mixed(p.precision) ? c->val32 : nullptr,
-            (p.precision == ISPH_PREC_MIXED_V64 || p.precision == ISPH_PREC_MIXED_MG) ? c->rsum
-                                                                                     : nullptr,
+            (p.precision == ISPH_PREC_MIXED_V64 || prec_mg(p.precision)) ? c->rsum : nullptr,
             c->dpos, c->rhs, c->ss, c->err);
     HIPCHK(c, hipGetLastError());
     c->assembled = true;
@@ -562,8 +566,9 @@ int solve_mixed_t(isph_ctx* c, const TI* nbr, TX* const* v, typename Pair2<TX>::
         if (c->sc_host->done == DONE_CONVERGED) return ISPH_OK;
         if (c->sc_host->done == DONE_FAILED)
             return fail(c, ISPH_E_NOCONV, "mixed-precision PPE solve did not converge");
-        if (c->sc_host->brk) {
-            // breakdown: restart the recurrence from the true residual, rhat = r
+        if (c->sc_host->brk || c->prm.precision == ISPH_PREC_IR) {
+            // breakdown, or iterative refinement: the next correction starts
+            // from zero on the true residual, rhat = r
             k_mx_start<TX><<<grid, BLOCK, 0, c->stream>>>(c->n, r64, rq, v[0], v[4], c->part, c->sc);
             k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_START, grid, c->part, c->sc, ctl);
             first = true;
@@ -618,7 +623,7 @@ int run_solve(isph_ctx* c) {
         return ISPH_OK;
     }
     HIPCHK(c, hipMemcpyAsync(x, s.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
-    int rc = c->prm.precision == ISPH_PREC_MIXED_MG ? solve_mg(c)
+    int rc = prec_mg(c->prm.precision) ? solve_mg(c)
              : mixed                                ? solve_mixed(c)
                                                     : solve_fp64(c);
     int rc2 = read_sc(c);

@@ -208,7 +208,8 @@ int solve_mg_on(isph_ctx* c, const MGSystem& sys);
 // restarted GMRES, the reference's live solve (isph_gmres.hip); converged is
 // false when the iteration limit was reached first
 int solve_gmres(isph_ctx* c, double* x, bool& converged);
-bool prec_mixed(int precision);  // the modes with FP32 matrix values
+bool prec_mixed(int precision);  // the modes the assembly gives FP32 values (and row sums)
+bool prec_mg(int precision);     // the multigrid-preconditioned modes
 void gm_free(isph_ctx* c);
 
 // discrete adjoint (isph_adjoint.hip)

@@ -1,5 +1,7 @@
-// isph_mg.hip -- multigrid-preconditioned mixed-precision PPE solve
-// (ISPH_PREC_MIXED_MG).
+// isph_mg.hip -- multigrid-preconditioned PPE solve: mixed precision
+// (ISPH_PREC_MIXED_MG, the V-cycle and the outer products in FP32 values)
+// and the same algorithm with every value FP64 (ISPH_PREC_FP64_MG), the
+// baseline that isolates the effect of precision.
 //
 // BiCGSTAB with right preconditioning by one V-cycle of an aggregation
 // multigrid built on the particle cell grid:
@@ -25,7 +27,8 @@
 // solution by reliable updates against the FP64 true residual, which is
 // also the stop test (||b - A64 x|| <= tol ||b||).  The scalars follow the
 // same protocol as the unpreconditioned solves and are finished by
-// k_finish.  Only the preconditioner is FP32.
+// k_finish.  Only the preconditioner and the matrix values are FP32; in
+// ISPH_PREC_FP64_MG they are FP64 (kernels templated on the value type T).
 //
 // The hierarchy is rebuilt for every solve (the matrix changes every step):
 // two kernels per level, one thread per coarse cell summing its children in
@@ -50,7 +53,11 @@ constexpr int MG_BLOCK_CYCLES = 2;  // block V-cycles per coarse solve
 // aggregates under-correct, and 1.5 cuts the BiCGSTAB iterations of the
 // 1024x1024 tank from 94 to 30 (1.3: 34; 1.7 in the block: 50)
 constexpr float MG_OVER = 1.5f;
-constexpr float MG_OMEGA = 0.7f;      // Jacobi damping
+// Jacobi damping 0.7, rounded to the V-cycle's value type T
+template <class T>
+__device__ __forceinline__ T mg_omega() { return (T)0.7; }
+template <>
+__device__ __forceinline__ float mg_omega<float>() { return 0.7f; }
 
 __device__ __forceinline__ int mg_col(int e, int) { return e; }
 __device__ __forceinline__ int mg_col(short e, int i) { return i + (int)e; }
@@ -67,10 +74,11 @@ __global__ void k_mg_cells(int n, GridInfo g, const int* __restrict__ key, int* 
 // A1 = P^T A P: per cell, its rows' entries summed by the neighbour's cell
 // offset; the unit diagonal adds 1 per row.  An entry outside the 3x3 cells
 // raises ERR_DOMAIN (cannot happen for lists built from the cell stencil).
+template <class T>
 __global__ void k_mg_galerkin1(GridInfo g, const int* __restrict__ bstart, size_t ld,
                                const int* __restrict__ nbr, const int* __restrict__ cnt,
                                const double* __restrict__ val, const int* __restrict__ cell,
-                               float* __restrict__ A1, uint32_t* err) {
+                               T* __restrict__ A1, uint32_t* err) {
     const int nc = g.nx * g.ny;
     int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= nc) return;
@@ -93,12 +101,13 @@ __global__ void k_mg_galerkin1(GridInfo g, const int* __restrict__ bstart, size_
             acc[(dx + 1) * 3 + (dy + 1)] += val[o];
         }
     }
-    for (int d = 0; d < 9; ++d) A1[(size_t)d * nc + c] = (float)acc[d];
+    for (int d = 0; d < 9; ++d) A1[(size_t)d * nc + c] = (T)acc[d];
 }
 
 // A_{l+1} from A_l: coarse cell C = (X, Y) owns the fine cells (2X+a, 2Y+b)
-__global__ void k_mg_galerkin(int nx, int ny, const float* __restrict__ A, int nx2, int ny2,
-                              float* __restrict__ A2) {
+template <class T>
+__global__ void k_mg_galerkin(int nx, int ny, const T* __restrict__ A, int nx2, int ny2,
+                              T* __restrict__ A2) {
     const int nc = nx * ny, nc2 = nx2 * ny2;
     int C = blockIdx.x * BLOCK + threadIdx.x;
     if (C >= nc2) return;
@@ -116,26 +125,26 @@ __global__ void k_mg_galerkin(int nx, int ny, const float* __restrict__ A, int n
                 acc[D] += (double)A[(size_t)d * nc + c];
             }
         }
-    for (int d = 0; d < 9; ++d) A2[(size_t)d * nc2 + C] = (float)acc[d];
+    for (int d = 0; d < 9; ++d) A2[(size_t)d * nc2 + C] = (T)acc[d];
 }
 
 // ---------------------------------------------------------------------------
-// V-cycle kernels (FP32)
+// V-cycle kernels (value type T: float for MIXED_MG, double for FP64_MG)
 // ---------------------------------------------------------------------------
 // Level 0 down.  The right-hand side b is formed from the outer pairs:
 // MODE 0: b = p = r + beta q (r on a restart), from rq = (r, q);
 // MODE 1: b = s = r - alpha v, from rv = (r, v).  b goes to bout (FP64,
 // the outer recurrences need it); x0 = omega b (Jacobi from 0, unit
 // diagonal) and r0 = b - A x0 = b - omega A b.
-template <class TI, int MODE, bool FIRST>
+template <class T, class TI, int MODE, bool FIRST>
 __global__ void __launch_bounds__(BLOCK) k_mg_fine_down(int n, size_t ld, const TI* __restrict__ nbr,
                                                         const int* __restrict__ cnt,
-                                                        const float* __restrict__ val,
+                                                        const T* __restrict__ val,
                                                         const double2* __restrict__ pairs,
                                                         const SolverScalars* sc,
                                                         double* __restrict__ bout,
-                                                        float* __restrict__ x0,
-                                                        float* __restrict__ r0) {
+                                                        T* __restrict__ x0,
+                                                        T* __restrict__ r0) {
     if (sc->done) return;
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -147,20 +156,21 @@ __global__ void __launch_bounds__(BLOCK) k_mg_fine_down(int n, size_t ld, const 
     };
     const double bi = bj(i);
     bout[i] = bi;
-    const float fb = (float)bi;
-    float acc = fb;
+    const T fb = (T)bi;
+    T acc = fb;
     const int c = cnt[i];
     for (int k = 0; k < c; ++k) {
         const size_t o = (size_t)k * ld + i;
-        acc += val[o] * (float)bj(mg_col(nbr[o], i));
+        acc += val[o] * (T)bj(mg_col(nbr[o], i));
     }
-    x0[i] = MG_OMEGA * fb;
-    r0[i] = fb - MG_OMEGA * acc;
+    x0[i] = mg_omega<T>() * fb;
+    r0[i] = fb - mg_omega<T>() * acc;
 }
 
 // restriction to the cells: b1[c] = sum of r0 over the cell's particles
+template <class T>
 __global__ void k_mg_restrict1(GridInfo g, const int* __restrict__ bstart,
-                               const float* __restrict__ r0, float* __restrict__ b1,
+                               const T* __restrict__ r0, T* __restrict__ b1,
                                const SolverScalars* sc) {
     if (sc->done) return;
     const int nc = g.nx * g.ny;
@@ -169,7 +179,7 @@ __global__ void k_mg_restrict1(GridInfo g, const int* __restrict__ bstart,
     const int cx = c / g.ny, cy = c - cx * g.ny;
     int lo, hi;
     column_bins(g, cx, cy, cy, lo, hi);
-    float s = 0.0f;
+    T s = (T)0;
     for (int i = bstart[lo]; i < bstart[hi]; ++i) s += r0[i];
     b1[c] = s;
 }
@@ -179,10 +189,11 @@ __global__ void k_mg_restrict1(GridInfo g, const int* __restrict__ bstart,
 // them), so a clamped neighbour adds 0 and the sweeps need no branches
 __device__ __forceinline__ int mg_clamp(int e, int n) { return min(max(e, 0), n - 1); }
 
-__device__ __forceinline__ float mg_jac0(const float* __restrict__ A, int nc, int c,
-                                         const float* __restrict__ b) {
-    const float dg = A[(size_t)4 * nc + c];
-    return dg != 0.0f ? MG_OMEGA * b[c] / dg : 0.0f;
+template <class T>
+__device__ __forceinline__ T mg_jac0(const T* __restrict__ A, int nc, int c,
+                                     const T* __restrict__ b) {
+    const T dg = A[(size_t)4 * nc + c];
+    return dg != (T)0 ? mg_omega<T>() * b[c] / dg : (T)0;
 }
 
 // grid level down and restriction: four consecutive lanes take the four
@@ -190,20 +201,21 @@ __device__ __forceinline__ float mg_jac0(const float* __restrict__ A, int nc, in
 // (2X + k/2, 2Y + k%2); each smooths its child from zero (x = omega D^-1 b)
 // and forms its residual r = b - A x; b2[C] = (r0 + r1) + (r2 + r3), the
 // same sum in every lane of the quad (missing children add 0)
-__global__ void k_mg_down(int nx, int ny, const float* __restrict__ A,
-                          const float* __restrict__ b, float* __restrict__ x, int nx2, int ny2,
-                          float* __restrict__ b2, const SolverScalars* sc) {
+template <class T>
+__global__ void k_mg_down(int nx, int ny, const T* __restrict__ A,
+                          const T* __restrict__ b, T* __restrict__ x, int nx2, int ny2,
+                          T* __restrict__ b2, const SolverScalars* sc) {
     if (sc->done) return;
     const int nc = nx * ny;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     const int C = t >> 2, k = t & 3;
-    float r = 0.0f;
+    T r = (T)0;
     if (C < nx2 * ny2) {
         const int X = C / ny2, Y = C - X * ny2;
         const int cx = 2 * X + (k >> 1), cy = 2 * Y + (k & 1);
         if (cx < nx && cy < ny) {
             const int c = cx * ny + cy;
-            float acc = 0.0f;
+            T acc = (T)0;
 #pragma unroll
             for (int d = 0; d < 9; ++d) {
                 const int ex = mg_clamp(cx + d / 3 - 1, nx), ey = mg_clamp(cy + d % 3 - 1, ny);
@@ -213,8 +225,8 @@ __global__ void k_mg_down(int nx, int ny, const float* __restrict__ A,
             r = b[c] - acc;
         }
     }
-    const float s = r + __shfl_xor(r, 1);
-    const float q = s + __shfl_xor(s, 2);
+    const T s = r + __shfl_xor(r, 1);
+    const T q = s + __shfl_xor(s, 2);
     if (k == 0 && C < nx2 * ny2) b2[C] = q;
 }
 
@@ -230,24 +242,26 @@ struct MGBlock {
     float over;                   // factor on the coarse correction
     int nx[MG_CLEV], ny[MG_CLEV];
     int off[MG_CLEV];             // offset of level l >= 1 in the lower-level arrays
-    const float* A[MG_CLEV];
+    const void* A[MG_CLEV];       // stencils, of the V-cycle's value type
 };
 
-__global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __restrict__ b,
-                                                      float* __restrict__ x,
+template <class T>
+__global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __restrict__ b,
+                                                      T* __restrict__ x,
                                                       const SolverScalars* sc) {
     if (sc->done) return;
-    __shared__ float xt[2][MG_CMAX];   // top level: iterate (two parities), residual
-    __shared__ float rt[MG_CMAX];
-    __shared__ float As[9 * MG_CLOW];  // lower levels: stencil d of cell c at 9 off + d n + c
-    __shared__ float xl[2][MG_CLOW];
-    __shared__ float bl[MG_CLOW];
+    __shared__ T xt[2][MG_CMAX];   // top level: iterate (two parities), residual
+    __shared__ T rt[MG_CMAX];
+    __shared__ T As[9 * MG_CLOW];  // lower levels: stencil d of cell c at 9 off + d n + c
+    __shared__ T xl[2][MG_CLOW];
+    __shared__ T bl[MG_CLOW];
+    const T omega = mg_omega<T>();
     const int c = threadIdx.x;
     const int K = B.K;
     const int ny0 = B.ny[0], n0 = B.nx[0] * ny0;
     const bool own = c < n0;
     const int cx0 = own ? c / ny0 : 0, cy0 = own ? c - cx0 * ny0 : 0;
-    float a[9], w = 0.0f, bc = 0.0f;
+    T a[9], w = (T)0, bc = (T)0;
     int nb[9], pa[9];   // neighbour cells and their parents on level 1
     int pc = 0;
     bool live = false;
@@ -256,47 +270,47 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __
         for (int d = 0; d < 9; ++d) {
             const int ex = cx0 + d / 3 - 1, ey = cy0 + d % 3 - 1;
             const bool in = ex >= 0 && ex < B.nx[0] && ey >= 0 && ey < ny0;
-            a[d] = in ? B.A[0][(size_t)d * n0 + c] : 0.0f;
+            a[d] = in ? static_cast<const T*>(B.A[0])[(size_t)d * n0 + c] : (T)0;
             nb[d] = in ? ex * ny0 + ey : c;
             pa[d] = in ? (ex >> 1) * ny1 + (ey >> 1) : 0;
         }
         pc = (cx0 >> 1) * ny1 + (cy0 >> 1);
-        live = a[4] != 0.0f;
-        w = live ? MG_OMEGA / a[4] : 0.0f;
+        live = a[4] != (T)0;
+        w = live ? omega / a[4] : (T)0;
         bc = b[c];
     }
     for (int l = 1; l < K; ++l) {
         const int n9 = 9 * B.nx[l] * B.ny[l];
-        const float* __restrict__ src = B.A[l];
-        float* dst = As + 9 * B.off[l];
+        const T* __restrict__ src = static_cast<const T*>(B.A[l]);
+        T* dst = As + 9 * B.off[l];
         for (int e = c; e < n9; e += MG_CMAX) dst[e] = src[e];
     }
     // LDS keeps what the last kernel on this CU left there: every iterate
     // starts from zero, and a sweep reads the coarser level's iterate only
     // when it applies its correction
-    xt[0][c] = 0.0f;
+    xt[0][c] = (T)0;
     for (int e = c; e < MG_CLOW; e += MG_CMAX) {
-        xl[0][e] = 0.0f;
-        xl[1][e] = 0.0f;
+        xl[0][e] = (T)0;
+        xl[1][e] = (T)0;
     }
     __syncthreads();
 
     // top sweep from parity p (z = x + e(parent) on level 1 at parity q if corr)
     auto top_sweep = [&](int p, bool corr, int q) {
         if (own) {
-            const float* xc = xt[p];
-            float acc = 0.0f;
-            float zi;
+            const T* xc = xt[p];
+            T acc = (T)0;
+            T zi;
             if (corr) {
-                const float* e = xl[q] + B.off[1];
-                const float f = B.over;
+                const T* e = xl[q] + B.off[1];
+                const T f = (T)B.over;
                 for (int d = 0; d < 9; ++d) acc += a[d] * (xc[nb[d]] + f * e[pa[d]]);
                 zi = xc[c] + f * e[pc];
             } else {
                 for (int d = 0; d < 9; ++d) acc += a[d] * xc[nb[d]];
                 zi = xc[c];
             }
-            xt[p ^ 1][c] = live ? zi + w * (bc - acc) : 0.0f;
+            xt[p ^ 1][c] = live ? zi + w * (bc - acc) : (T)0;
         }
         __syncthreads();
     };
@@ -305,21 +319,21 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __
     auto low_sweep = [&](int l, int p, bool zero, bool corr, int q) {
         const int nx = B.nx[l], ny = B.ny[l], n = nx * ny, o = B.off[l];
         if (c < n) {
-            const float* A = As + 9 * o;
-            const float dg = A[4 * n + c];
-            float nv = 0.0f;
-            if (dg != 0.0f) {
+            const T* A = As + 9 * o;
+            const T dg = A[4 * n + c];
+            T nv = (T)0;
+            if (dg != (T)0) {
                 if (zero) {
-                    nv = MG_OMEGA * bl[o + c] / dg;
+                    nv = omega * bl[o + c] / dg;
                 } else {
                     const int cx = c / ny, cy = c - cx * ny;
-                    const float* xc = xl[p] + o;
-                    float acc = 0.0f;
-                    float zi;
+                    const T* xc = xl[p] + o;
+                    T acc = (T)0;
+                    T zi;
                     if (corr) {
                         const int ny2 = B.ny[l + 1];
-                        const float* e = xl[q] + B.off[l + 1];
-                        const float f = B.over;
+                        const T* e = xl[q] + B.off[l + 1];
+                        const T f = (T)B.over;
 #pragma unroll
                         for (int d = 0; d < 9; ++d) {
                             const int ex = mg_clamp(cx + d / 3 - 1, nx);
@@ -336,7 +350,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __
                         }
                         zi = xc[c];
                     }
-                    nv = zi + MG_OMEGA * (bl[o + c] - acc) / dg;
+                    nv = zi + omega * (bl[o + c] - acc) / dg;
                 }
             }
             xl[zero ? 1 : p ^ 1][o + c] = nv;
@@ -350,15 +364,15 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __
         const int ny2 = B.ny[l + 1];
         if (c < B.nx[l + 1] * ny2) {
             const int X = c / ny2, Y = c - X * ny2;
-            const float* A = As + 9 * o;
-            const float* xc = xl[p] + o;
-            float s = 0.0f;
+            const T* A = As + 9 * o;
+            const T* xc = xl[p] + o;
+            T s = (T)0;
             for (int u = 0; u < 2; ++u)
                 for (int v = 0; v < 2; ++v) {
                     const int cx = 2 * X + u, cy = 2 * Y + v;
                     if (cx >= nx || cy >= ny) continue;
                     const int cc = cx * ny + cy;
-                    float acc = 0.0f;
+                    T acc = (T)0;
 #pragma unroll
                     for (int d = 0; d < 9; ++d) {
                         const int ex = mg_clamp(cx + d / 3 - 1, nx), ey = mg_clamp(cy + d % 3 - 1, ny);
@@ -372,7 +386,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __
     };
 
     if (K == 1) {  // a single cell: exact
-        if (own) x[c] = live ? bc / a[4] : 0.0f;
+        if (own) x[c] = live ? bc / a[4] : (T)0;
         return;
     }
     const int nu = B.nu;
@@ -383,7 +397,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __
             pt ^= 1;
         }
         if (own) {
-            float acc = 0.0f;
+            T acc = (T)0;
             for (int d = 0; d < 9; ++d) acc += a[d] * xt[pt][nb[d]];
             rt[c] = bc - acc;
         }
@@ -392,7 +406,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __
             const int ny1 = B.ny[1];
             if (c < B.nx[1] * ny1) {
                 const int X = c / ny1, Y = c - X * ny1;
-                float s = 0.0f;
+                T s = (T)0;
                 for (int u = 0; u < 2; ++u)
                     for (int v = 0; v < 2; ++v) {
                         const int cx = 2 * X + u, cy = 2 * Y + v;
@@ -416,8 +430,8 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __
         {
             const int o = B.off[K - 1];
             if (c == 0) {
-                const float dg = As[9 * o + 4];
-                xl[1][o] = dg != 0.0f ? bl[o] / dg : 0.0f;
+                const T dg = As[9 * o + 4];
+                xl[1][o] = dg != (T)0 ? bl[o] / dg : (T)0;
             }
             __syncthreads();
         }
@@ -444,63 +458,66 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const float* __
 }
 
 // coarse correction of level 0: x0 += over e1(cell)
-__global__ void k_mg_prolong1(int n, const int* __restrict__ cell, const float* __restrict__ e1,
-                              float over, float* __restrict__ x0, const SolverScalars* sc) {
+template <class T>
+__global__ void k_mg_prolong1(int n, const int* __restrict__ cell, const T* __restrict__ e1,
+                              float over, T* __restrict__ x0, const SolverScalars* sc) {
     if (sc->done) return;
     int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) x0[i] += over * e1[cell[i]];
+    if (i < n) x0[i] += (T)over * e1[cell[i]];
 }
 
 // grid level up: z = x + over e(parent), out = z + omega D^-1 (b - A z)
-__global__ void k_mg_grid_up(int nx, int ny, const float* __restrict__ A,
-                             const float* __restrict__ b, const float* __restrict__ x, int ny2,
-                             const float* __restrict__ e, float over, float* __restrict__ out,
+template <class T>
+__global__ void k_mg_grid_up(int nx, int ny, const T* __restrict__ A,
+                             const T* __restrict__ b, const T* __restrict__ x, int ny2,
+                             const T* __restrict__ e, float over, T* __restrict__ out,
                              const SolverScalars* sc) {
     if (sc->done) return;
     const int nc = nx * ny;
     int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= nc) return;
     const int cx = c / ny, cy = c - cx * ny;
-    auto z = [&](int ex, int ey) { return x[ex * ny + ey] + over * e[(ex >> 1) * ny2 + (ey >> 1)]; };
-    float acc = 0.0f;
+    const T f = (T)over;
+    auto z = [&](int ex, int ey) { return x[ex * ny + ey] + f * e[(ex >> 1) * ny2 + (ey >> 1)]; };
+    T acc = (T)0;
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
         const int ex = mg_clamp(cx + d / 3 - 1, nx), ey = mg_clamp(cy + d % 3 - 1, ny);
         acc += A[(size_t)d * nc + c] * z(ex, ey);
     }
-    const float dg = A[(size_t)4 * nc + c];
-    const float zc = z(cx, cy);
-    out[c] = dg != 0.0f ? zc + MG_OMEGA * (b[c] - acc) / dg : 0.0f;
+    const T dg = A[(size_t)4 * nc + c];
+    const T zc = z(cx, cy);
+    out[c] = dg != (T)0 ? zc + mg_omega<T>() * (b[c] - acc) / dg : (T)0;
 }
 
 // level 0 up: z = x0 + e1(cell) (k_mg_prolong1), out = z + omega (b - A z)
-template <class TI>
+template <class T, class TI>
 __global__ void __launch_bounds__(BLOCK) k_mg_fine_up(int n, size_t ld, const TI* __restrict__ nbr,
                                                       const int* __restrict__ cnt,
-                                                      const float* __restrict__ val,
+                                                      const T* __restrict__ val,
                                                       const double* __restrict__ b,
-                                                      const float* __restrict__ z,
+                                                      const T* __restrict__ z,
                                                       double* __restrict__ out,
                                                       const SolverScalars* sc) {
     if (sc->done) return;
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    const float zi = z[i];
-    float acc = zi;
+    const T zi = z[i];
+    T acc = zi;
     const int c = cnt[i];
     for (int k = 0; k < c; ++k) {
         const size_t o = (size_t)k * ld + i;
         acc += val[o] * z[mg_col(nbr[o], i)];
     }
-    out[i] = (double)(zi + MG_OMEGA * ((float)b[i] - acc));
+    out[i] = (double)(zi + mg_omega<T>() * ((T)b[i] - acc));
 }
 
 // ---------------------------------------------------------------------------
 // outer BiCGSTAB kernels (FP64 vectors, partials per XCD-mapped block)
 // ---------------------------------------------------------------------------
-template <class TI>
+template <class TI, class TV>
 __device__ __forceinline__ double mg_spmv(int i, int c, double si, const TI* __restrict__ nbr,
-                                          const float* __restrict__ val, size_t ld,
+                                          const TV* __restrict__ val, size_t ld,
                                           const double* __restrict__ x) {
     const double xi = x[i];
     double acc = 0.0;
@@ -512,10 +529,10 @@ __device__ __forceinline__ double mg_spmv(int i, int c, double si, const TI* __r
 }
 
 // v = A p^; rv = (r, v); partial (v, rhat)
-template <class TI>
+template <class TI, class TV>
 __global__ void __launch_bounds__(BLOCK) k_mg_v(int n, size_t ld, const TI* __restrict__ nbr,
                                                 const int* __restrict__ cnt,
-                                                const float* __restrict__ val,
+                                                const TV* __restrict__ val,
                                                 const double* __restrict__ rsum,
                                                 const double* __restrict__ ph,
                                                 const double2* __restrict__ rq,
@@ -527,7 +544,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_v(int n, size_t ld, const TI* __re
     const int i = lb * BLOCK + threadIdx.x;
     double v = 0.0;
     if (i < n) {
-        const double s = mg_spmv<TI>(i, cnt[i], rsum[i], nbr, val, ld, ph);
+        const double s = mg_spmv<TI, TV>(i, cnt[i], rsum[i], nbr, val, ld, ph);
         rv[i] = make_double2(rq[i].x, s);
         v = s * rhat[i];
     }
@@ -536,10 +553,10 @@ __global__ void __launch_bounds__(BLOCK) k_mg_v(int n, size_t ld, const TI* __re
 }
 
 // t = A s^; partials (t, s), (t, t)
-template <class TI>
+template <class TI, class TV>
 __global__ void __launch_bounds__(BLOCK) k_mg_t(int n, size_t ld, const TI* __restrict__ nbr,
                                                 const int* __restrict__ cnt,
-                                                const float* __restrict__ val,
+                                                const TV* __restrict__ val,
                                                 const double* __restrict__ rsum,
                                                 const double* __restrict__ sh,
                                                 const double* __restrict__ s,
@@ -550,7 +567,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_t(int n, size_t ld, const TI* __re
     const int i = lb * BLOCK + threadIdx.x;
     double v0 = 0.0, v1 = 0.0;
     if (i < n) {
-        const double ti = mg_spmv<TI>(i, cnt[i], rsum[i], nbr, val, ld, sh);
+        const double ti = mg_spmv<TI, TV>(i, cnt[i], rsum[i], nbr, val, ld, sh);
         t[i] = ti;
         v0 = ti * s[i];
         v1 = ti * ti;
@@ -617,9 +634,11 @@ __global__ void k_mg_lower(int n, size_t ld, const int* __restrict__ cnt,
 
 namespace host {
 
+// Level arrays are allocated with 8-byte elements and read as the solve's
+// value type (float or double).
 struct MGLevel {
     int nx = 0, ny = 0;
-    float *A = nullptr, *b = nullptr, *x = nullptr, *x2 = nullptr;
+    double *A = nullptr, *b = nullptr, *x = nullptr, *x2 = nullptr;
 };
 
 struct MGData {
@@ -627,9 +646,20 @@ struct MGData {
     int lb = 0;    // first level solved inside one block (k_mg_block)
     MGBlock blk;
     MGLevel lv[MG_MAXLEV];
-    float *x0 = nullptr, *r0 = nullptr;
+    double *x0 = nullptr, *r0 = nullptr;
     int* cell = nullptr;
 };
+
+template <class T>
+static inline T* as(double* p) { return reinterpret_cast<T*>(p); }
+
+// the matrix values the V-cycle and the outer products read
+template <class T>
+static inline const T* mg_vals(const MGSystem& sys);
+template <>
+inline const float* mg_vals<float>(const MGSystem& sys) { return sys.val32; }
+template <>
+inline const double* mg_vals<double>(const MGSystem& sys) { return sys.val64; }
 
 void mg_free(isph_ctx* c) {
     MGData* m = c->mg;
@@ -705,19 +735,20 @@ static int mg_alloc(isph_ctx* c) {
     return ISPH_OK;
 }
 
-// Galerkin hierarchy of the system's matrix
+// Galerkin hierarchy of the system's matrix (FP64 sums, stored as T)
+template <class T>
 static int mg_setup(isph_ctx* c, const MGSystem& sys) {
     int rc = mg_alloc(c);
     if (rc) return rc;
     MGData* m = c->mg;
     const GridInfo& g = c->grid;
     k_mg_cells<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, g, c->key_s, m->cell);
-    k_mg_galerkin1<<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
-        g, c->bincnt, (size_t)c->cap, c->nbr, c->cnt, sys.val64, m->cell, m->lv[0].A, c->err);
+    k_mg_galerkin1<T><<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
+        g, c->bincnt, (size_t)c->cap, c->nbr, c->cnt, sys.val64, m->cell, as<T>(m->lv[0].A), c->err);
     for (int l = 0; l + 1 < m->nlev; ++l) {
         const MGLevel &a = m->lv[l], &b = m->lv[l + 1];
-        k_mg_galerkin<<<nblk((int64_t)b.nx * b.ny), BLOCK, 0, c->stream>>>(a.nx, a.ny, a.A, b.nx,
-                                                                            b.ny, b.A);
+        k_mg_galerkin<T><<<nblk((int64_t)b.nx * b.ny), BLOCK, 0, c->stream>>>(
+            a.nx, a.ny, as<T>(a.A), b.nx, b.ny, as<T>(b.A));
     }
     HIPCHK(c, hipGetLastError());
     return ISPH_OK;
@@ -725,40 +756,43 @@ static int mg_setup(isph_ctx* c, const MGSystem& sys) {
 
 // one V-cycle: b formed from `pairs` (MODE 0: p from rq, MODE 1: s from rv)
 // into bout, preconditioned vector into out
-template <class TI, int MODE, bool FIRST>
+template <class T, class TI, int MODE, bool FIRST>
 static void mg_vcycle(isph_ctx* c, const MGSystem& sys, const TI* nbr, const double2* pairs,
                       double* bout, double* out) {
     MGData* m = c->mg;
     const GridInfo& g = c->grid;
     const int n = c->n;
     const size_t ld = (size_t)c->cap;
-    k_mg_fine_down<TI, MODE, FIRST><<<nblk(n), BLOCK, 0, c->stream>>>(
-        n, ld, nbr, c->cnt, sys.val32, pairs, c->sc, bout, m->x0, m->r0);
-    k_mg_restrict1<<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(g, c->bincnt, m->r0,
-                                                                         m->lv[0].b, c->sc);
+    const T* val = mg_vals<T>(sys);
+    k_mg_fine_down<T, TI, MODE, FIRST><<<nblk(n), BLOCK, 0, c->stream>>>(
+        n, ld, nbr, c->cnt, val, pairs, c->sc, bout, as<T>(m->x0), as<T>(m->r0));
+    k_mg_restrict1<T><<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
+        g, c->bincnt, as<T>(m->r0), as<T>(m->lv[0].b), c->sc);
     const int lb = m->lb;
     for (int l = 0; l < lb; ++l) {
         MGLevel &a = m->lv[l], &b = m->lv[l + 1];
-        k_mg_down<<<nblk(4 * (int64_t)b.nx * b.ny), BLOCK, 0, c->stream>>>(
-            a.nx, a.ny, a.A, a.b, a.x, b.nx, b.ny, b.b, c->sc);
+        k_mg_down<T><<<nblk(4 * (int64_t)b.nx * b.ny), BLOCK, 0, c->stream>>>(
+            a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.nx, b.ny, as<T>(b.b), c->sc);
     }
-    k_mg_block<<<1, MG_CMAX, 0, c->stream>>>(m->blk, m->lv[lb].b, m->lv[lb].x, c->sc);
+    k_mg_block<T><<<1, MG_CMAX, 0, c->stream>>>(m->blk, as<T>(m->lv[lb].b), as<T>(m->lv[lb].x),
+                                                 c->sc);
     for (int l = lb - 1; l >= 0; --l) {
         MGLevel &a = m->lv[l], &b = m->lv[l + 1];
-        k_mg_grid_up<<<nblk((int64_t)a.nx * a.ny), BLOCK, 0, c->stream>>>(a.nx, a.ny, a.A, a.b, a.x,
-                                                                           b.ny, b.x, MG_OVER, a.x2,
-                                                                           c->sc);
+        k_mg_grid_up<T><<<nblk((int64_t)a.nx * a.ny), BLOCK, 0, c->stream>>>(
+            a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.ny, as<T>(b.x), MG_OVER, as<T>(a.x2),
+            c->sc);
         std::swap(a.x, a.x2);
     }
-    k_mg_prolong1<<<nblk(n), BLOCK, 0, c->stream>>>(n, m->cell, m->lv[0].x, MG_OVER, m->x0,
-                                                    c->sc);
-    k_mg_fine_up<TI><<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, sys.val32, bout, m->x0,
-                                                        out, c->sc);
+    k_mg_prolong1<T><<<nblk(n), BLOCK, 0, c->stream>>>(n, m->cell, as<T>(m->lv[0].x), MG_OVER,
+                                                       as<T>(m->x0), c->sc);
+    k_mg_fine_up<T, TI><<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, val, bout,
+                                                           as<T>(m->x0), out, c->sc);
 }
 
-template <class TI>
+template <class T, class TI>
 static int mg_iterate(isph_ctx* c, const MGSystem& sys, const TI* nbr, const SolveCtl& ctl,
                       bool& first) {
+    const T* val = mg_vals<T>(sys);
     const int grid = nblk8(c->n);
     const int n = c->n;
     const size_t ld = (size_t)c->cap;
@@ -776,15 +810,15 @@ static int mg_iterate(isph_ctx* c, const MGSystem& sys, const TI* nbr, const Sol
         }
         for (int b = 0; b < batch; ++b) {
             if (first)
-                mg_vcycle<TI, 0, true>(c, sys, nbr, c->rq64, p, ph);
+                mg_vcycle<T, TI, 0, true>(c, sys, nbr, c->rq64, p, ph);
             else
-                mg_vcycle<TI, 0, false>(c, sys, nbr, c->rq64, p, ph);
-            k_mg_v<TI><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, sys.val32, sys.rsum, ph,
-                                                       c->rq64, rhat, c->rv64, c->part, c->sc);
+                mg_vcycle<T, TI, 0, false>(c, sys, nbr, c->rq64, p, ph);
+            k_mg_v<TI, T><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, val, sys.rsum, ph,
+                                                          c->rq64, rhat, c->rv64, c->part, c->sc);
             k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_ALPHA, grid, c->part, c->sc, ctl);
-            mg_vcycle<TI, 1, false>(c, sys, nbr, c->rv64, s, sh);
-            k_mg_t<TI><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, sys.val32, sys.rsum, sh, s,
-                                                       t, c->part, c->sc);
+            mg_vcycle<T, TI, 1, false>(c, sys, nbr, c->rv64, s, sh);
+            k_mg_t<TI, T><<<grid, BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, val, sys.rsum, sh, s,
+                                                          t, c->part, c->sc);
             k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_OMEGA, grid, c->part, c->sc, ctl);
             k_mg_x<<<grid, BLOCK, 0, c->stream>>>(n, p, ph, s, sh, t, c->rv64, rhat, xs, c->rq64,
                                                    c->part, c->sc);
@@ -798,12 +832,12 @@ static int mg_iterate(isph_ctx* c, const MGSystem& sys, const TI* nbr, const Sol
     }
 }
 
-template <class TI>
+template <class T, class TI>
 static int solve_mg_t(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
     const int grid = nblk8(c->n);
     double* x = sys.x;
     double* r64 = c->v64[9];
-    int rc = mg_setup(c, sys);
+    int rc = mg_setup<T>(c, sys);
     if (rc) return rc;
     SolveCtl ctl = solve_ctl(c, 1);
     SolveCtl ctl_flush = solve_ctl(c, 2);
@@ -825,7 +859,7 @@ static int solve_mg_t(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
     if ((rc = read_sc(c))) return rc;
     bool first = true;
     for (;;) {
-        if ((rc = mg_iterate<TI>(c, sys, nbr, ctl, first))) return rc;
+        if ((rc = mg_iterate<T, TI>(c, sys, nbr, ctl, first))) return rc;
         const int d = c->sc_host->done;
         if (d == DONE_FAILED) return fail(c, ISPH_E_NOCONV, "multigrid PPE solve did not converge");
         if (d != DONE_UPDATE) return fail(c, ISPH_E_STATE, "unexpected solver state");
@@ -865,7 +899,11 @@ int solve_mg_on(isph_ctx* c, const MGSystem& sys) {
     const bool narrow = !(e & ERR_INDEX16);
     if (!narrow) HIPCHK(c, hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->stream));
     c->narrow_idx = narrow;
-    return narrow ? solve_mg_t<short>(c, sys, c->nbr16) : solve_mg_t<int>(c, sys, c->nbr);
+    if (c->prm.precision == ISPH_PREC_FP64_MG)
+        return narrow ? solve_mg_t<double, short>(c, sys, c->nbr16)
+                      : solve_mg_t<double, int>(c, sys, c->nbr);
+    return narrow ? solve_mg_t<float, short>(c, sys, c->nbr16)
+                  : solve_mg_t<float, int>(c, sys, c->nbr);
 }
 
 }  // namespace host

@@ -15,7 +15,8 @@ module isph_capi
     integer(c_int), parameter :: ISPH_KERNEL_HYBRID = 0, ISPH_KERNEL_WENDLAND_C2 = 1
     integer(c_int), parameter :: ISPH_DRAG_REFERENCE = 0, ISPH_DRAG_ERGUN = 1
     integer(c_int), parameter :: ISPH_PREC_FP64 = 0, ISPH_PREC_MIXED = 1, ISPH_PREC_MIXED_V64 = 2
-    integer(c_int), parameter :: ISPH_PREC_MIXED_MG = 3, ISPH_PREC_FGMRES = 4
+    integer(c_int), parameter :: ISPH_PREC_MIXED_MG = 3, ISPH_PREC_FGMRES = 4, ISPH_PREC_IR = 5
+    integer(c_int), parameter :: ISPH_PREC_FP64_MG = 6
 
     ! same member order as the C struct: doubles first, then int32s
     type, bind(C) :: isph_params

@@ -17,6 +17,8 @@ PREC_MIXED = 1
 PREC_MIXED_V64 = 2
 PREC_MIXED_MG = 3
 PREC_FGMRES = 4
+PREC_IR = 5
+PREC_FP64_MG = 6
 VISC_REFERENCE = 0
 VISC_CONSTANT = 1
 ORDER_CELL = 0

@@ -69,6 +69,16 @@ extern "C" {
 #define ISPH_PREC_MIXED_MG  3  /* as MIXED_V64, right-preconditioned by one FP32
                                   V-cycle of an aggregation multigrid on the cell
                                   grid (cells, then 2x2 blocks of cells)             */
+#define ISPH_PREC_IR        5  /* iterative refinement: FP64 outer residual
+                                  b - A64 x and update x += d; each correction d
+                                  solves A d = r with FP32 matrix values and an
+                                  FP32-vector BiCGSTAB (FP64 dot products) started
+                                  from zero (rhat = r, p = r) and stopped when its
+                                  recurrence residual fell by inner_tol             */
+#define ISPH_PREC_FP64_MG   6  /* MIXED_MG's algorithm with every value FP64: the
+                                  FP64 matrix in the outer products and an FP64
+                                  V-cycle on FP64 Galerkin operators (the baseline
+                                  that isolates what FP32 buys in MIXED_MG)         */
 #define ISPH_PREC_FGMRES    4  /* FP64 restarted GMRES, the reference's live solve
                                   (solver.f90:306-347, MKL dfgmres without a
                                   preconditioner): relative tolerance `tol` on the

@@ -307,7 +307,7 @@ static int check_params(ora_ctx* c, const isph_params* prm) {
         return set_err(c, ISPH_E_ARG, "unknown cell_order");
     if (prm->cell_order == ISPH_ORDER_REFERENCE && !(prm->cell_size >= 4.0 * prm->h))
         return set_err(c, ISPH_E_ARG, "the reference order needs cell_size >= 4h");
-    if (prm->precision < ISPH_PREC_FP64 || prm->precision > ISPH_PREC_FGMRES)
+    if (prm->precision < ISPH_PREC_FP64 || prm->precision > ISPH_PREC_FP64_MG)
         return set_err(c, ISPH_E_ARG, "unknown precision");
     int32_t nx, ny;
     grid_dims(prm, &nx, &ny);
@@ -1351,7 +1351,7 @@ static int NAME(ora_ctx* c, const float* val32, double* x, int32_t* iters, int32
             }                                                                               \
             for (int32_t i = 0; i < n; ++i) rs[i] = (TX)r64[i];                             \
             rupd = sqrt(tr);                                                                \
-            if (brk) { /* full restart: rhat = r */                                         \
+            if (brk || prm->precision == ISPH_PREC_IR) { /* restart from zero: rhat = r */  \
                 for (int32_t i = 0; i < n; ++i) res[i] = rs[i];                             \
                 prevr = DOT(res, rs, n);                                                    \
                 prevr_prev = prevr;                                                         \
@@ -1531,7 +1531,7 @@ int ora_ppe_solve(ora_ctx* c, int32_t* iters, double* relres) {
     int32_t it = 0, outer = 0;
     double rel = 0.0;
     int rc;
-    if (prm->precision == ISPH_PREC_MIXED_MG)
+    if (prm->precision == ISPH_PREC_MIXED_MG || prm->precision == ISPH_PREC_FP64_MG)
         return set_err(c, ISPH_E_ARG, "the multigrid solve is checked against the FP64 solve, "
                                       "it has no restatement here");
     if (prm->precision == ISPH_PREC_FGMRES) {
@@ -1551,7 +1551,8 @@ int ora_ppe_solve(ora_ctx* c, int32_t* iters, double* relres) {
         memcpy(c->p, c->fsol, sizeof(double) * (size_t)c->n);
         return ISPH_OK;
     }
-    if (prm->precision == ISPH_PREC_MIXED || prm->precision == ISPH_PREC_MIXED_V64) {
+    if (prm->precision == ISPH_PREC_MIXED || prm->precision == ISPH_PREC_MIXED_V64 ||
+        prm->precision == ISPH_PREC_IR) {
         rc = mixed_solve(c, x, &it, &outer, &rel);
     } else {
         rc = bicgstab_core(c->n, c->rowptr, c->col, c->val, c->rhs, x, prm->tol, prm->maxit,
