@@ -53,13 +53,15 @@ def parse():
                     help="skip the 4M-particle multigrid step")
     ap.add_argument("--mg-only", action="store_true",
                     help="time only the multigrid-preconditioned step (for profiling)")
+    ap.add_argument("--fp64mg-only", action="store_true",
+                    help="time only the FP64-valued multigrid step (for profiling)")
     ap.add_argument("--fgmres-only", action="store_true",
                     help="time only the step with the reference's FGMRES solve (for profiling)")
     ap.add_argument("--reference-run-only", action="store_true")
     ap.add_argument("--adjoint-dx", type=float, default=0.002)
     ap.add_argument("--adjoint-steps", type=int, default=50)
-    ap.add_argument("--cpu-iters", type=int, default=2000,
-                    help="BiCGSTAB iterations timed on the CPU oracle")
+    ap.add_argument("--cpu-runs", type=int, default=3,
+                    help="runs of each CPU-oracle workload (the median is reported)")
     ap.add_argument("--pmc-run", action="store_true",
                     help="short run for hardware-counter profiles: one step whose FP64 solve is "
                          "capped at 40 iterations, then 5 copies of 1 GiB with the library's "
@@ -117,17 +119,24 @@ def run_gpu(cfg, steps, warmup, precision, profile, tol=None, maxit=200000, list
     if profile:
         sim.profile_enable(PROFILE_STRIDE)
     import torch
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    # each step is timed on its own, bracketed by device synchronisations;
+    # the restore of the input (a device-to-device copy, synchronous) lies
+    # outside the timed region and is reported apart
+    seconds = restore_s = 0.0
     for _ in range(steps):
+        r0 = time.perf_counter()
         sim.restore(0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         sim.step(1)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        seconds += t1 - t0
+        restore_s += t0 - r0
         iters.append(sim.scalars()["iters"])
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
     s = sim.scalars()
-    out = dict(sim=sim, seconds=t1 - t0, iters=iters, relres=s["relres"], outer=s["outer"],
-               first=first)
+    out = dict(sim=sim, seconds=seconds, restore_s=restore_s, iters=iters, relres=s["relres"],
+               outer=s["outer"], first=first)
     if lists:
         ptr, idx = sim.get_neighbours()
         rows = np.repeat(np.arange(len(ptr) - 1), np.diff(ptr))
@@ -161,16 +170,22 @@ def scalart_bytes(ptr, tid):
     return int(104 * fl.sum() + 4 * cnt[fl].sum() + 8 * (~fl).sum())
 
 
-def run_salinity(dx, steps, warmup):
+SALT_DT = 2.5e-5     # the dam break at dx = 0.001 is stable at this dt (not at 1e-4)
+SALT_WARMUP, SALT_STEPS = 2, 10
+
+
+def run_salinity(dx, steps=SALT_STEPS, warmup=SALT_WARMUP, dt=SALT_DT):
     """The salt-water step (main.f90:199-263) on the porous dam break at
-    spacing dx: salt water above the line y - 0.1 = 0.3 (x - 0.2) in the
-    column, fresh below; transport substeps on the previous step's lists,
-    densityupdate, the lagged eddyvis.  Steps run back to back (the
-    transport reads the lists of the previous step)."""
+    spacing dx and time step dt: salt water above the line
+    y - 0.1 = 0.3 (x - 0.2) in the column, fresh below; transport substeps
+    on the previous step's lists, densityupdate, the lagged eddyvis.  Steps
+    run back to back (the transport reads the lists of the previous step),
+    so their number is fixed here, independent of --steps: the run must
+    stay physical for all of them."""
     import torch
     from beach_slope_confined_amd import capi, configs
     cfg = configs.dam_break(dx)
-    cfg["params"] = cfg["params"].copy(salinity=1)
+    cfg["params"] = cfg["params"].copy(salinity=1, dt=dt)
     sim = configs.make_sim(cfg, capi.IsphSim)
     sim.salt_init(0.2, 0.1, 0.3)
     for _ in range(max(warmup, 1)):   # the first step builds the lists
@@ -200,7 +215,7 @@ def run_salinity(dx, steps, warmup):
     # profiling events; con compared particle by particle with the FP64 run
     from beach_slope_confined_amd.params import PREC_MIXED_MG
     cmg = configs.dam_break(dx)
-    cmg["params"] = cmg["params"].copy(salinity=1, precision=PREC_MIXED_MG, tol=1e-9)
+    cmg["params"] = cmg["params"].copy(salinity=1, precision=PREC_MIXED_MG, tol=1e-9, dt=dt)
     sim = configs.make_sim(cmg, capi.IsphSim)
     sim.salt_init(0.2, 0.1, 0.3)
     for _ in range(max(warmup, 1)):
@@ -221,8 +236,8 @@ def run_salinity(dx, steps, warmup):
           "max_con_diff_vs_fp64": dcon}
     sim.close()
     return {"workload": f"dam_break dx={dx:g}: {n_fluid} fluid, {n} particles, "
-                        f"{len(cfg['soil']['x'])} soil; salt-water step, salt above "
-                        f"y - 0.1 = 0.3 (x - 0.2)",
+                        f"{len(cfg['soil']['x'])} soil; salt-water step at dt={dt:g}, salt "
+                        f"above y - 0.1 = 0.3 (x - 0.2); {warmup} + {steps} steps",
             "ppe_iters_last_step": iters,
             "value": round(n_fluid * steps / (t1 - t0), 1),
             "ms_per_step": round(1e3 * (t1 - t0) / steps, 3), "substeps_per_step": sub,
@@ -373,45 +388,63 @@ def run_adjoint(dx, nsteps, every, multigrid=False, dt=None):
             "adjoint_bicgstab_iters": int(adj_iters)}
 
 
-def cpu_baseline(cfg, gpu_iters):
-    """Oracle step on the host cores: every stage once at full size plus
-    `cpu_iters` BiCGSTAB iterations; the solve is scaled to the GPU's
-    iteration count (the oracle's iterates are bit-identical to the GPU's)."""
+def host_cores():
+    """Physical cores of the host (unique core/socket pairs from lscpu), the
+    CPUs this process may run on, and the OpenMP threads the oracle uses."""
+    import subprocess
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True,
+                             check=True).stdout
+        phys = len({ln for ln in out.splitlines() if ln and not ln.startswith("#")})
+    except Exception:
+        pass
+    allowed = len(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", allowed))
+    return phys, allowed, min(threads, allowed)
+
+
+def cpu_baseline(cfg, runs):
+    """The CPU oracle (this repository's C/OpenMP restatement of the
+    reference step) on the host cores: the full single step at scale (every
+    stage, BiCGSTAB to the same 1e-8 true residual; its iterates are
+    bit-identical to the GPU's), `runs` times from the same input, median;
+    and the small dam break (C0 geometry, 2,556 particles) for 200 steps,
+    `runs` times, median."""
     from beach_slope_confined_amd import configs
     from oracle import pyoracle
     from oracle.pyoracle import OracleSim
-    threads = pyoracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-    sim = configs.make_sim(cfg, OracleSim)
-    t = {}
-
-    def timed(name, fn):
+    phys, allowed, want = host_cores()
+    threads = pyoracle.set_threads(want)
+    big = []
+    for _ in range(runs):
+        sim = configs.make_sim(cfg, OracleSim)
         a = time.perf_counter()
-        fn()
-        t[name] = t.get(name, 0.0) + time.perf_counter() - a
-
-    s = sim.scalars()
-    sim.set_scalars(s["umax"], s["dt"], s["t"] + s["dt"])
-    for name in ("projection", "cellshift", "resetid", "neighbour", "effpor", "eddyvis",
-                 "int_vel", "ppe_assemble"):
-        timed(name, getattr(sim, name))
-    prm = sim.params.copy(maxit=args.cpu_iters)
-    sim.set_params(prm)
-    a = time.perf_counter()
-    try:
-        sim.ppe_solve()
-    except Exception:
-        pass  # stops at maxit by design
-    t_iter = (time.perf_counter() - a) / args.cpu_iters
-    for name in ("comp_vel", "comp_pos", "cellshiftalt", "neighbour", "freesurf", "effpor",
-                 "opt2_shift", "timestep"):
-        timed(name + "_2" if name in t else name, getattr(sim, name))
-    fixed = sum(t.values())
-    step = fixed + gpu_iters * t_iter
-    sample = (f"oracle stages once at {len(cfg['parts']['x'])} particles ({fixed:.2f} s) + "
-              f"{args.cpu_iters} BiCGSTAB iterations ({t_iter * 1e3:.1f} ms each), solve scaled "
-              f"to the GPU's {gpu_iters} iterations")
+        sim.step(1)
+        big.append(time.perf_counter() - a)
+        it = sim.scalars()["iters"]
+        sim.close()
+    small = []
+    c0 = configs.dam_break(0.01)
+    for _ in range(runs):
+        sim = configs.make_sim(c0, OracleSim)
+        a = time.perf_counter()
+        sim.step(200)
+        small.append(time.perf_counter() - a)
+        sim.close()
+    step = float(np.median(big))
+    s200 = float(np.median(small))
+    sample = (f"oracle: the full step at {len(cfg['parts']['x'])} particles ({it} BiCGSTAB "
+              f"iterations), {runs} runs from the same input, median {step:.2f} s (runs "
+              + ", ".join(f"{v:.2f}" for v in big) + f" s); dam break dx=0.01 "
+              f"({len(c0['parts']['x'])} particles) for 200 steps, {runs} runs, median "
+              f"{s200:.3f} s")
     return dict(value=cfg["n_fluid"] / step, unit="particle-steps/s", cores=threads,
-                kind="port", sample=sample, seconds_per_step=step)
+                kind="port", sample=sample, seconds_per_step=step, runs_s=big,
+                host_physical_cores=phys, host_cpus_allowed=allowed,
+                c0_200_steps={"seconds": s200, "runs_s": small,
+                              "particle_steps_per_s": c0["n_fluid"] * 200 / s200,
+                              "n_fluid": c0["n_fluid"], "n_total": len(c0["parts"]["x"])})
 
 
 def main():
@@ -425,8 +458,8 @@ def main():
     import torch
     torch.cuda.init()
     from beach_slope_confined_amd import configs
-    from beach_slope_confined_amd.params import (PREC_FGMRES, PREC_FP64, PREC_MIXED, PREC_MIXED_MG,
-                                                 PREC_MIXED_V64)
+    from beach_slope_confined_amd.params import (PREC_FGMRES, PREC_FP64, PREC_FP64_MG, PREC_IR,
+                                                 PREC_MIXED, PREC_MIXED_MG, PREC_MIXED_V64)
 
     cfg = configs.tank_1m(args.nside)
     if args.pmc_run:
@@ -449,7 +482,7 @@ def main():
               flush=True)
         return
     if args.salinity_only:
-        sal = run_salinity(args.salt_dx, args.steps, args.warmup)
+        sal = run_salinity(args.salt_dx)
         sal["k_scalart_tank"] = run_scalart_at_scale(cfg)
         print(json.dumps({"salinity": sal}), flush=True)
         return
@@ -463,10 +496,12 @@ def main():
             "iters_per_step": rgm["iters"][args.warmup:], "relres": rgm["relres"]}}), flush=True)
         return
 
-    if args.mg_only:
-        rmg = run_gpu(cfg, args.steps, args.warmup, PREC_MIXED_MG, False, tol=1e-9)
+    if args.mg_only or args.fp64mg_only:
+        key = "mixed_mg" if args.mg_only else "fp64_mg"
+        rmg = run_gpu(cfg, args.steps, args.warmup,
+                      PREC_MIXED_MG if args.mg_only else PREC_FP64_MG, False, tol=1e-9)
         rmg["sim"].close()
-        print(json.dumps({"mixed_mg": {
+        print(json.dumps({key: {
             "value": round(n_fluid * args.steps / rmg["seconds"], 1),
             "ms_per_step": round(1e3 * rmg["seconds"] / args.steps, 3), "tol": 1e-9,
             "iters_per_step": rmg["iters"][args.warmup:], "relres": rmg["relres"]}}), flush=True)
@@ -527,6 +562,7 @@ def main():
                    "seq_len": 1, "parallelism": "single GPU",
                    "kernel": "wendland_c2", "h_over_dx": 1.3, "tol": 1e-8},
         "ppe_iters_per_step": timed_iters, "relres": r64["relres"],
+        "restore_ms_per_step_untimed": round(1e3 * r64["restore_s"] / max(args.steps, 1), 3),
         "particle_steps_per_s_total": round(n_total * args.steps / r64["seconds"], 1),
         "roofline": roofline, "step_model": step_model,
     }
@@ -601,6 +637,49 @@ def main():
             except Exception as e:
                 line["mixed_mg"]["tight_error"] = str(e)
 
+    if not args.no_mixed and "value" in line.get("mixed_mg", {}):
+        # the same multigrid algorithm with every value FP64: mixed_mg over
+        # fp64_mg is what FP32 buys at an equal true residual (1e-9)
+        try:
+            rf = run_gpu(cfg, args.steps, args.warmup, PREC_FP64_MG, False, tol=1e-9)
+            rf["sim"].close()
+            vf = n_fluid * args.steps / rf["seconds"]
+            pe, ve = step_errors(r64["first"], rf["first"])
+            mg = line["mixed_mg"]
+            line["fp64_mg"] = {
+                "value": round(vf, 1), "ms_per_step": round(1e3 * rf["seconds"] / args.steps, 3),
+                "speedup_vs_fp64": round(vf / value, 3), "tol": 1e-9,
+                "iters_per_step": rf["iters"][args.warmup:], "relres": rf["relres"],
+                "pressure_err_vs_fp64": pe, "velocity_err_vs_fp64": ve,
+            }
+            if "tight_tol" in mg:
+                pt, vt = step_errors(rtight["first"], rf["first"])
+                line["fp64_mg"].update(pressure_err_vs_tight=pt, velocity_err_vs_tight=vt)
+            pm, vm = step_errors(rf["first"], rmg["first"])
+            mg["speedup_vs_fp64_mg"] = round(mg["value"] / vf, 3)
+            mg["pressure_err_vs_fp64_mg"] = pm
+            mg["velocity_err_vs_fp64_mg"] = vm
+        except Exception as e:  # reported, not hidden
+            line["fp64_mg"] = {"error": str(e)}
+        # iterative refinement with an FP32 inner BiCGSTAB restarted from
+        # zero on every FP64 true residual, to the FP64 bar (1e-8)
+        try:
+            rir = run_gpu(cfg, args.steps, args.warmup, PREC_IR, False)
+            rir["sim"].close()
+            vir = n_fluid * args.steps / rir["seconds"]
+            pe, ve = step_errors(r64["first"], rir["first"])
+            line["ir"] = {
+                "value": round(vir, 1), "ms_per_step": round(1e3 * rir["seconds"] / args.steps, 3),
+                "speedup_vs_fp64": round(vir / value, 3), "tol": 1e-8,
+                "iters_per_step": rir["iters"][args.warmup:], "corrections": rir["outer"],
+                "relres": rir["relres"], "pressure_err_vs_fp64": pe, "velocity_err_vs_fp64": ve,
+            }
+        except Exception as e:  # reported, not hidden
+            line["ir"] = {"error": str(e),
+                          "note": "iterative refinement with an unpreconditioned FP32 inner "
+                                  "BiCGSTAB restarted from zero: a measured negative result "
+                                  "when it fails"}
+
     if not args.no_mixed:
         # the reference's live solve (FGMRES, tol 1e-3 on the residual
         # estimate, at most 150 iterations, warm start fsol) on the same input
@@ -663,7 +742,7 @@ def main():
 
     if not args.no_salinity:
         try:
-            line["salinity"] = run_salinity(args.salt_dx, args.steps, args.warmup)
+            line["salinity"] = run_salinity(args.salt_dx)
             line["salinity"]["k_scalart_tank"] = run_scalart_at_scale(cfg)
         except Exception as e:  # reported, not hidden
             line["salinity"] = {"error": str(e)}
@@ -694,7 +773,7 @@ def main():
             line["long_run"] = {"error": str(e)}
 
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(cfg, int(np.mean(timed_iters)))
+        line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_runs)
     print(json.dumps(line), flush=True)
 
 
