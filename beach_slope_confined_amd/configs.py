@@ -176,6 +176,52 @@ def beach_large(side=2048, dx=0.0005, seed=SEED, jitter=0.05):
                 n_fluid=nf)
 
 
+def beach_wide(ncol=8192, nrow=512, dx=0.0005, seed=SEED, jitter=0.05, dt=1.25e-5):
+    """The large beach run as a wide, shallow dam break: a water column of
+    ncol x nrow particles (4,194,304 at the default) at spacing dx, the dam
+    break's proportions about the column's height Hc = nrow dx (tank 4 Hc
+    longer than the column and 2 Hc high, the toe of the 1:3 porous slope
+    2 Hc / 3 inside the column, so the slope stands 2 Hc / 9 high at the
+    column's end, as in the dam break), porosity 0.4, d50 0.01, alpha 200,
+    beta 1.1.  A depth of 512 dx keeps H/dx near the dam break's at
+    dx = 0.001 (300), the finest that runs 500 steps, where the square
+    2048 x 2048 column (H/dx = 2048) leaves the grid in its first steps."""
+    w, hc = ncol * dx, nrow * dx
+    L, H = w + 4.0 * hc, 2.0 * hc
+    toe = w - 2.0 * hc / 3.0
+    nx, ny = int(round(L / dx)), int(round(H / dx))
+    ii, jj = np.meshgrid(np.arange(ncol), np.arange(nrow), indexing="ij")
+    ii = ii.ravel().astype(np.float64)
+    jj = jj.ravel().astype(np.float64)
+    nf = len(ii)
+    jit = jitter * dx
+    fx = (ii + 0.5) * dx + (2.0 * uniform(nf, 1, seed) - 1.0) * jit
+    fy = (jj + 0.5) * dx + (2.0 * uniform(nf, 2, seed) - 1.0) * jit
+    wx, wy, wtid = _walls(nx, ny, dx, 0.0, 0.0, top=True)
+    rho0, g = 1000.0, -9.81
+    fp = rho0 * abs(g) * np.maximum(hc - fy, 0.0)
+    parts = _assemble(fx, fy, np.zeros(nf), np.zeros(nf), wx, wy, wtid, rho0, dx, fp,
+                      np.zeros(len(wx)))
+    por = 0.4
+    in_pores = np.concatenate([np.zeros(len(wx), bool), fy < (fx - toe) / 3.0])
+    parts["mass"][in_pores] = por * rho0 * dx * dx
+    # soil lattice under the slope only where the slope is (x > toe)
+    i0 = int(toe / dx)
+    si, sj = np.meshgrid(np.arange(i0, nx), np.arange(ny), indexing="ij")
+    sx = (si.ravel() + 0.5) * dx
+    sy = (sj.ravel() + 0.5) * dx
+    keep = sy < (sx - toe) / 3.0
+    sx, sy = sx[keep], sy[keep]
+    soil = dict(x=sx, y=sy, mass=np.full(len(sx), 1800.0 * dx * dx),
+                density=np.full(len(sx), 1800.0))
+    m = 3.5 * dx
+    prm = make_params(kernel=KERNEL_WENDLAND_C2, drag_law=DRAG_ERGUN, visc_law=VISC_CONSTANT,
+                      h=1.3 * dx, dl=dx, dt=dt, por=por, alpha=200.0, beta=1.1, d50=0.01,
+                      xmin=-m, ymin=-m, xmax=L + m, ymax=H + m)
+    return dict(name=f"beach_wide_{ncol}x{nrow}_dx{dx:g}", parts=parts, soil=soil, params=prm,
+                n_fluid=nf, toe=toe, column=(w, hc))
+
+
 def fourier_velocity(x, y, seed=SEED, modes=16, amp=0.1, noise=1e-3):
     """vx = sum_k (amp/modes) cos(2 pi (kx x + ky y) + phi_k), vy likewise with
     sin, integer wavevectors in [1, 8]^2, plus U(-noise, noise)."""
