@@ -198,12 +198,18 @@ def load_library(path=LIB_PATH):
              ctypes.c_int),
             ("probe_noalloc", [ctypes.c_void_p, ctypes.c_int32, _c_dp, _c_dp, _c_dp],
              ctypes.c_int),
+            ("set_progress", [ctypes.c_void_p, PROGRESS_FN, ctypes.c_void_p], ctypes.c_int),
         ]:
             f = getattr(lib, "isph_" + name)
             f.argtypes = args
             f.restype = res
         _LIB = lib
     return _LIB
+
+
+# void fn(void* user, const char* what, int32_t done, int32_t total)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32,
+                               ctypes.c_int32)
 
 
 class Sim:
@@ -433,6 +439,18 @@ class IsphSim(Sim):
     def restore(self, slot):
         self._check(self._lib.isph_restore(self._h, int(slot)), "restore")
         self.n = self.count()
+
+    def set_progress(self, fn):
+        """fn(what, done, total) after each checkpoint segment of the adjoint
+        forward and backward sweeps; None turns it off."""
+        if fn is None:
+            self._progress = None
+            cb = ctypes.cast(None, PROGRESS_FN)
+        else:
+            self._progress = PROGRESS_FN(lambda user, what, done, total:
+                                         fn(what.decode(), done, total))
+            cb = self._progress
+        self._check(self._lib.isph_set_progress(self._h, cb, None), "set_progress")
 
     def profile_enable(self, stride=1):
         """Time every `stride`-th SpMV launch with HIP events (0: off)."""

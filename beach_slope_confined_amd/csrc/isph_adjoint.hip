@@ -589,6 +589,10 @@ int isph_adjoint_forward(isph_ctx* c, const isph_objective* obj, int32_t nsteps,
             return rc;
         }
         if ((rc = obj_sums(c, S->oc, c->st[c->cur], weight, true))) return rc;
+        if (c->progress && ((k + 1) % every == 0 || k + 1 == nsteps)) {
+            if ((rc = sync_check(c))) return rc;
+            c->progress(c->progress_user, "adjoint_forward", k / every + 1, nseg);
+        }
     }
     prof_collect(c);
     if ((rc = sync_check(c))) return rc;
@@ -647,6 +651,7 @@ int isph_adjoint_backward(isph_ctx* c, double* grad3, double* J, int64_t* adj_it
                                              : (g + 1 < nseg ? S->seg[g + 1].v : S->fin.v);
             if ((rc = reverse_step(c, S->tape[s], end, weight, &iters))) return rc;
         }
+        if (c->progress) c->progress(c->progress_user, "adjoint_backward", nseg - g, nseg);
     }
     // initial porous closure
     if ((rc = adj_effpor(c, S->A[S->ca], S->seg[0].v))) return rc;

@@ -1266,6 +1266,13 @@ int isph_restore(isph_ctx* c, int32_t slot) {
     return ISPH_OK;
 }
 
+int isph_set_progress(isph_ctx* c, isph_progress_fn fn, void* user) {
+    if (!c) return ISPH_E_ARG;
+    c->progress = fn;
+    c->progress_user = user;
+    return ISPH_OK;
+}
+
 int isph_profile_enable(isph_ctx* c, int32_t enable) {
     if (!c || enable < 0) return ISPH_E_ARG;
     c->prof.on = enable != 0;

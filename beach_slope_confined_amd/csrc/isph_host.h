@@ -122,6 +122,8 @@ struct isph_ctx {
     double zone_shift = 0.0;
     int64_t next_pid = 0;          // pid of the next inserted particle (int64: max pid + 1 may exceed int)
     double* probe_scratch = nullptr;  // this GPU's isph_probe_noalloc scratch (device address)
+    isph_progress_fn progress = nullptr;  // isph_set_progress
+    void* progress_user = nullptr;
     std::string msg;
 };
 

@@ -35,6 +35,11 @@ HBM_PEAK_GBS = 8000.0
 PROFILE_STRIDE = 16  # HIP events around every 16th SpMV launch of the timed steps
 
 
+def log(msg):
+    """Progress on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,7 +55,11 @@ def parse():
     ap.add_argument("--salt-dx", type=float, default=0.001)
     ap.add_argument("--no-reference-run", action="store_true")
     ap.add_argument("--no-large", action="store_true",
-                    help="skip the 4M-particle multigrid step")
+                    help="skip the 4M-particle beach run")
+    ap.add_argument("--large-steps", type=int, default=500,
+                    help="steps of the 4M-particle beach run")
+    ap.add_argument("--no-large-adjoint", dest="large_adjoint", action="store_false",
+                    help="skip the adjoint of the 4M-particle beach run")
     ap.add_argument("--mg-only", action="store_true",
                     help="time only the multigrid-preconditioned step (for profiling)")
     ap.add_argument("--fp64mg-only", action="store_true",
@@ -104,6 +113,7 @@ def run_gpu(cfg, steps, warmup, precision, profile, tol=None, maxit=200000, list
     c = dict(cfg)
     c["params"] = cfg["params"].copy(precision=precision, tol=tol or cfg["params"].tol,
                                      maxit=maxit)
+    log(f"{cfg['name']}: precision {precision}, {warmup} + {steps} steps")
     sim = configs.make_sim(c, capi.IsphSim)
     # every step starts from the same device-resident input (the workload is
     # one ISPH step at scale; the snapshot copy is device-to-device)
@@ -404,6 +414,142 @@ def host_cores():
     return phys, allowed, min(threads, allowed)
 
 
+def model_mg_step_bytes(n, nnz_off, iters, val_bytes=4):
+    """Algorithmic bytes of one multigrid-preconditioned step: the fixed
+    stages 1284 + 60 K per row (the FP64 step's model) plus, per BiCGSTAB
+    iteration, two V-cycles (the particle-level down and up passes each
+    read the row's offsets and values once, 6 K with FP32 values, plus
+    36 and 24 bytes of own vectors; restriction, prolongation and the cell
+    levels about 26 bytes), the two outer products A p^, A s^ (6 K + 60,
+    6 K + 36) and the update pass (96): 36 K + 364 per row with FP32 values.
+    With FP64 values (val_bytes 8) each of the six matrix passes reads
+    4 K more and the V-cycle's own vectors 8 more."""
+    k = nnz_off / n
+    extra = (val_bytes - 4) * (6 * k + 2)
+    return n * ((1284 + 60 * k) + iters * (36 * k + 364 + extra))
+
+
+def run_large(steps, with_adjoint):
+    """The large beach run: 4,194,304 fluid particles (an 8192 x 512 column
+    at dx = 0.0005 with the dam break's 1:3 porous slope, porosity 0.4,
+    d50 0.01, alpha 200, beta 1.1; configs.beach_wide), `steps` multigrid
+    steps with shifting at dt = 1.25e-5 from rest, then the adjoint of the
+    whole run (checkpoints every 10 steps) with respect to (por, alpha,
+    beta).  The column starts on a perfect lattice: with the +-0.05 dx
+    jitter of the small dam break, the reference's symmetric pressure
+    gradient is unstable at a depth of 512 particles (the run leaves the
+    grid at step 4, at dx = 0.0005 and at dx = 0.001 alike; 256 is stable).
+    Beside it, from the same input: one step with the FP64-valued
+    multigrid solve (the same algorithm, FP64 values), the error of the
+    first step's pressure against a tight FP64 solve (FP64 multigrid to
+    1e-11), and one plain FP64 BiCGSTAB step (the drop-in) within 40,000
+    iterations."""
+    import torch
+    from beach_slope_confined_amd import adjoint as adj
+    from beach_slope_confined_amd import capi, configs
+    from beach_slope_confined_amd.params import PREC_FP64, PREC_FP64_MG, PREC_MIXED_MG
+    big = configs.beach_wide(dt=1.25e-5, jitter=0.0)
+    nf, nt, ns = big["n_fluid"], len(big["parts"]["x"]), len(big["soil"]["x"])
+    out = {"workload": f"{big['name']}: {nf} fluid, {nt} particles, {ns} soil; multigrid PPE "
+                       f"(tol 1e-9), dt 1.25e-5, shifting, from rest on a perfect lattice",
+           "n_fluid": nf, "n_total": nt, "n_soil": ns, "steps": steps, "tol": 1e-9}
+
+    def first_step(prec, tol, maxit=40000, warm=True):
+        c = dict(big)
+        c["params"] = big["params"].copy(precision=prec, tol=tol, maxit=maxit)
+        sim = configs.make_sim(c, capi.IsphSim)
+        if warm:
+            sim.checkpoint(0)
+            sim.step(1)        # warm-up (allocations, the multigrid hierarchy)
+            sim.restore(0)
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        sim.step(1)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - a)
+        s = sim.scalars()
+        q = sim.get_particles()
+        sim.close()
+        return ms, s, q
+
+    log(f"{big['name']}: first steps (mixed and FP64 multigrid, tight, plain FP64)")
+    ms_mx, s_mx, q_mx = first_step(PREC_MIXED_MG, 1e-9)
+    ms_64, s_64, q_64 = first_step(PREC_FP64_MG, 1e-9)
+    # 1e-11: the FP64 true residual of 4.26M rows does not reach 1e-12
+    _, s_t, q_t = first_step(PREC_FP64_MG, 1e-11)
+    pe, ve = step_errors(q_t, q_mx)
+    pe64, ve64 = step_errors(q_t, q_64)
+    out["first_step"] = {
+        "mixed_mg_ms": round(ms_mx, 3), "mixed_mg_iters": s_mx["iters"],
+        "fp64_mg_ms": round(ms_64, 3), "fp64_mg_iters": s_64["iters"],
+        "mixed_mg_speedup_vs_fp64_mg": round(ms_64 / ms_mx, 3),
+        "tight": "FP64 multigrid to 1e-11, %d iterations" % s_t["iters"],
+        "mixed_mg_pressure_err_vs_tight": pe, "mixed_mg_velocity_err_vs_tight": ve,
+        "fp64_mg_pressure_err_vs_tight": pe64, "fp64_mg_velocity_err_vs_tight": ve64}
+    try:
+        ms_p, s_p, q_p = first_step(PREC_FP64, 1e-8, warm=False)
+        pp, vp = step_errors(q_t, q_p)
+        out["first_step"]["fp64_plain"] = {"ms": round(ms_p, 3), "iters": s_p["iters"],
+                                           "pressure_err_vs_tight": pp,
+                                           "velocity_err_vs_tight": vp}
+    except Exception as e:  # reported, not hidden
+        out["first_step"]["fp64_plain"] = {"error": str(e), "maxit": 40000}
+    del q_mx, q_64, q_t
+
+    c = dict(big)
+    c["params"] = big["params"].copy(precision=PREC_MIXED_MG, tol=1e-9, maxit=40000)
+    sim = configs.make_sim(c, capi.IsphSim)
+    sim.checkpoint(0)
+    its = []
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    for k in range(steps):
+        sim.step(1)
+        its.append(sim.scalars()["iters"])
+        if (k + 1) % 100 == 0:
+            log(f"{big['name']}: {k + 1} of {steps} steps")
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - a
+    s = sim.scalars()
+    ptr, _ = sim.get_neighbours()
+    nnz = int(ptr[-1])
+    n = len(ptr) - 1
+    sb = model_mg_step_bytes(n, nnz, float(np.mean(its)))
+    out["forward"] = {"seconds": round(sec, 3), "ms_per_step": round(1e3 * sec / steps, 3),
+                      "value": round(nf * steps / sec, 1),
+                      "iters_mean": round(float(np.mean(its)), 2), "iters_max": int(max(its)),
+                      "umax_at_end": s["umax"], "t_at_end": s["t"]}
+    out["step_model"] = {"bytes_per_step": round(sb), "achieved": round(sb / (sec / steps) / 1e9, 1),
+                         "unit": "GB/s", "frac": round(sb / (sec / steps) / 1e9 / HBM_PEAK_GBS, 4),
+                         "model": "rows x (1284 + 60K + iterations x (36K + 364)) bytes, "
+                                  "K = %.3f" % (nnz / n)}
+    if with_adjoint:
+        w, _hc = big["column"]
+        toe = big["toe"]
+        obj = adj.slope_objective(toe=toe, probe_x=(toe + 0.01, toe + 0.02, toe + 0.03),
+                                  gauges=(w - 0.05, w - 0.5))
+        sim.set_progress(lambda what, done, total:
+                         log(f"{big['name']}: {what} {done} of {total} segments")
+                         if done % 10 == 0 or done == total else None)
+        sim.restore(0)
+        torch.cuda.synchronize()
+        a0 = time.perf_counter()
+        J = sim.adjoint_forward(obj, steps, 10)
+        torch.cuda.synchronize()
+        a1 = time.perf_counter()
+        g, _, it = sim.adjoint_backward()
+        torch.cuda.synchronize()
+        a2 = time.perf_counter()
+        out["adjoint"] = {"steps": steps, "checkpoint_every": 10,
+                          "objective": "mean over steps of 3 slope-face probes + 2 depth gauges",
+                          "forward_s": round(a1 - a0, 3), "backward_s": round(a2 - a1, 3),
+                          "adjoint_over_forward": round((a2 - a1) / (a1 - a0), 3), "J": J,
+                          "grad_por_alpha_beta": [float(v) for v in g],
+                          "transposed_iters": int(it)}
+    sim.close()
+    return out
+
+
 def cpu_baseline(cfg, runs):
     """The CPU oracle (this repository's C/OpenMP restatement of the
     reference step) on the host cores: the full single step at scale (every
@@ -417,11 +563,12 @@ def cpu_baseline(cfg, runs):
     phys, allowed, want = host_cores()
     threads = pyoracle.set_threads(want)
     big = []
-    for _ in range(runs):
+    for r in range(runs):
         sim = configs.make_sim(cfg, OracleSim)
         a = time.perf_counter()
         sim.step(1)
         big.append(time.perf_counter() - a)
+        log(f"oracle full step {r + 1} of {runs}: {big[-1]:.2f} s")
         it = sim.scalars()["iters"]
         sim.close()
     small = []
@@ -699,49 +846,14 @@ def main():
             line["fgmres_reference"] = {"error": str(e)}
 
     if not args.no_mixed and not args.no_large:
-        # the large beach (4,194,304 fluid + 73,764 wall particles, 8,562,630
-        # soil): one multigrid step from the same checkpoint, as the 1M step
         try:
-            big = configs.beach_large()
-            rbig = run_gpu(big, args.steps, args.warmup, PREC_MIXED_MG, False, tol=1e-9,
-                           lists=False)
-            # the adjoint of one multigrid step (the configuration does not
-            # survive a second step at this resolution): probes on this
-            # tank's slope face y = (x - toe)/3, gauge in the column
-            import torch
-            from beach_slope_confined_amd import adjoint as adj
-            sb = rbig["sim"]
-            toe = 0.5 * 2048 * 0.0005
-            obj = adj.slope_objective(toe=toe, probe_x=(toe + 0.05, toe + 0.1, toe + 0.15),
-                                      gauges=(0.5,))
-            sb.restore(0)
-            torch.cuda.synchronize()
-            a0 = time.perf_counter()
-            Jb = sb.adjoint_forward(obj, 1, 1)
-            torch.cuda.synchronize()
-            a1 = time.perf_counter()
-            gb, _, itb = sb.adjoint_backward()
-            torch.cuda.synchronize()
-            a2 = time.perf_counter()
-            sb.close()
-            nfb = big["n_fluid"]
-            line["large_mg"] = {
-                "workload": big["name"], "n_fluid": nfb, "n_total": len(big["parts"]["x"]),
-                "n_soil": len(big["soil"]["x"]),
-                "value": round(nfb * args.steps / rbig["seconds"], 1),
-                "ms_per_step": round(1e3 * rbig["seconds"] / args.steps, 3), "tol": 1e-9,
-                "iters_per_step": rbig["iters"][args.warmup:], "relres": rbig["relres"],
-                "adjoint_one_step": {"forward_s": round(a1 - a0, 4), "backward_s": round(a2 - a1, 4),
-                                     "adjoint_over_forward": round((a2 - a1) / (a1 - a0), 3),
-                                     "J": Jb, "grad_por_alpha_beta": [float(v) for v in gb],
-                                     "transposed_iters": int(itb)},
-            }
-            del big
+            line["large_mg"] = run_large(args.large_steps, args.large_adjoint)
         except Exception as e:  # reported, not hidden
             line["large_mg"] = {"error": str(e)}
 
     if not args.no_salinity:
         try:
+            log("salinity leg")
             line["salinity"] = run_salinity(args.salt_dx)
             line["salinity"]["k_scalart_tank"] = run_scalart_at_scale(cfg)
         except Exception as e:  # reported, not hidden
@@ -749,12 +861,14 @@ def main():
 
     if not args.no_reference_run:
         try:
+            log("reference scenario leg")
             line["reference_run"] = run_reference_scenario(20, 0 if args.no_cpu else 2)
         except Exception as e:  # reported, not hidden
             line["reference_run"] = {"error": str(e)}
 
     if not args.no_adjoint:
         try:
+            log("adjoint leg")
             line["adjoint"] = run_adjoint(args.adjoint_dx, args.adjoint_steps, 10)
             mg = run_adjoint(args.adjoint_dx, args.adjoint_steps, 10, multigrid=True)
             g64 = np.array(line["adjoint"]["grad_por_alpha_beta"])
@@ -768,11 +882,13 @@ def main():
         # the whole run, on the finest dam break that stays stable for that
         # long (dx = 0.001 at dt = 2.5e-5; the 4M beach does not)
         try:
+            log("long-run leg")
             line["long_run"] = run_adjoint(0.001, 500, 10, multigrid=True, dt=2.5e-5)
         except Exception as e:  # reported, not hidden
             line["long_run"] = {"error": str(e)}
 
     if not args.no_cpu:
+        log("CPU baseline (oracle)")
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_runs)
     print(json.dumps(line), flush=True)
 

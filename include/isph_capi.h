@@ -386,6 +386,13 @@ int isph_restore(isph_ctx* ctx, int32_t slot);
  * (p update, v = A p), 1 = FP64 step-2 SpMV (s, t = A s), 2 and 3 = the
  * same for the mixed-precision flavours, 4 = scalart. */
 int isph_profile_enable(isph_ctx* ctx, int32_t enable);
+/* Progress of the long calls: fn(user, what, done, total) is called on the
+ * calling thread after each checkpoint segment of isph_adjoint_forward
+ * (what = "adjoint_forward") and of isph_adjoint_backward
+ * ("adjoint_backward").  NULL turns it off.  fn must not call into the
+ * same context. */
+typedef void (*isph_progress_fn)(void* user, const char* what, int32_t done, int32_t total);
+int isph_set_progress(isph_ctx* ctx, isph_progress_fn fn, void* user);
 int isph_get_kernel_time(isph_ctx* ctx, int32_t kernel, int64_t* launches,
                          double* total_ms);
 /* Copies `bytes` bytes device-to-device with the library's copy kernel
