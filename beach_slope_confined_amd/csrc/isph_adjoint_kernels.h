@@ -194,7 +194,7 @@ __device__ inline void kgc_pair(const KernelConsts& kc, const T& pix, const T& p
 __global__ void __launch_bounds__(BLOCK) k_adj_kgc_self(int n, KernelConsts kc, size_t ld,
                                                         StateView s, const int* __restrict__ nbr,
                                                         const int* __restrict__ cnt, AdjView a) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     double lb[4];
     ld_L(a.L, i, lb);
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_kgc_self(int n, KernelConsts kc, 
 __global__ void __launch_bounds__(BLOCK) k_adj_kgc_nbr(int n, KernelConsts kc, size_t ld,
                                                        StateView s, const int* __restrict__ nbr,
                                                        const int* __restrict__ cnt, AdjView a) {
-    int j = blockIdx.x * BLOCK + threadIdx.x;
+    int j = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (j >= n) return;
     const double2 pj = s.pos[j];
     Dual<3> px = seed<3>(pj.x, 0), py = seed<3>(pj.y, 1), rj = seed<3>(s.rho[j], 2);
@@ -307,7 +307,7 @@ __global__ void k_adj_effpor(int n, GridInfo g, KernelConsts kc, PorousParams pp
                              const double2* __restrict__ spos, const double* __restrict__ sm,
                              const double* __restrict__ sr, const int* __restrict__ sstart,
                              AdjView a) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const double bR = a.R[i];
     a.R[i] = 0.0;  // R is overwritten for every particle
@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_strain_self(int n, KernelConsts k
                                                            const int* __restrict__ cnt,
                                                            const double* __restrict__ L,
                                                            const SimScalars* ss, AdjView a) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const size_t cp = a.cap;
     const bool act = eddy ? (s.tid[i] == 3) : ((s.flags[i] & ISPH_FLAG_FREE) != 0);
@@ -470,7 +470,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_strain_nbr(int n, KernelConsts kc
                                                           const int* __restrict__ nbr,
                                                           const int* __restrict__ cnt,
                                                           const double* __restrict__ L, AdjView a) {
-    int j = blockIdx.x * BLOCK + threadIdx.x;
+    int j = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (j >= n) return;
     const size_t cp = a.cap;
     const double2 pj = s.pos[j], vj = s.vel[j];
@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_vstar_self(int n, VstarConst vc, 
                                                           const int* __restrict__ cnt,
                                                           const double* __restrict__ L,
                                                           const SimScalars* ss, AdjView a) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const size_t cp = a.cap;
     const int fi = s.flags[i];
@@ -655,7 +655,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_vstar_nbr(int n, VstarConst vc, s
                                                          StateView s, const int* __restrict__ nbr,
                                                          const int* __restrict__ cnt,
                                                          const double* __restrict__ L, AdjView a) {
-    int j = blockIdx.x * BLOCK + threadIdx.x;
+    int j = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (j >= n) return;
     const size_t cp = a.cap;
     Dual<8> nj[8];
@@ -789,7 +789,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_asm_self(int n, AsmConst ac, size
                                                         const double* __restrict__ lam,
                                                         const double* __restrict__ p,
                                                         const SimScalars* ss, AdjView a) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const size_t cp = a.cap;
     const double dt = ss->dt;
@@ -841,7 +841,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_asm_nbr(int n, AsmConst ac, size_
                                                        const double* __restrict__ L,
                                                        const double* __restrict__ p,
                                                        const SimScalars* ss, AdjView a) {
-    int j = blockIdx.x * BLOCK + threadIdx.x;
+    int j = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (j >= n) return;
     const size_t cp = a.cap;
     const double dt = ss->dt;
@@ -921,7 +921,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_cv_self(int n, KernelConsts kc, s
                                                        const double* __restrict__ L,
                                                        const double* __restrict__ p,
                                                        const SimScalars* ss, AdjView a) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const size_t cp = a.cap;
     const int fi = s.flags[i];
@@ -995,7 +995,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_cv_nbr(int n, KernelConsts kc, si
                                                       const double* __restrict__ L,
                                                       const double* __restrict__ p,
                                                       const SimScalars* ss, AdjView a) {
-    int j = blockIdx.x * BLOCK + threadIdx.x;
+    int j = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (j >= n) return;
     const size_t cp = a.cap;
     const double dt = ss->dt;
@@ -1075,7 +1075,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_shift_self(int n, KernelConsts kc
                                                           const int* __restrict__ nbr,
                                                           const int* __restrict__ cnt,
                                                           const double* __restrict__ L, AdjView a) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const size_t cp = a.cap;
     if (!shift_row(s, cnt, i)) {
@@ -1131,7 +1131,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_shift_nbr(int n, KernelConsts kc,
                                                          const int* __restrict__ nbr,
                                                          const int* __restrict__ cnt,
                                                          const double* __restrict__ L, AdjView a) {
-    int j = blockIdx.x * BLOCK + threadIdx.x;
+    int j = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (j >= n) return;
     const size_t cp = a.cap;
     Dual<5> nj[5];
@@ -1301,7 +1301,7 @@ __global__ void k_adj_unperm(int n, const int* __restrict__ perm, AdjView d, Adj
 __global__ void k_transpose_vals(int n, size_t ld, const int* __restrict__ nbr,
                                  const int* __restrict__ cnt, const double* __restrict__ val,
                                  double* __restrict__ valT, uint32_t* err) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const int c = cnt[i];
     for (int k = 0; k < c; ++k) {

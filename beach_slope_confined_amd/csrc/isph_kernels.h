@@ -153,6 +153,18 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks) {
     return (bid & 7) * per + (bid >> 3);
 }
 
+// Row tile of physical block bid for a grid of any size nb: the hardware
+// deals workgroups to the 8 XCDs round-robin (bid & 7), so XCD x runs the
+// blocks x, x + 8, ...; this gives XCD x the contiguous logical range
+// [x q + min(x, r), ...) of q + (x < r) blocks (q = nb / 8, r = nb % 8).
+// A particle's neighbours lie within a few cell columns of it in the sorted
+// order, so with contiguous ranges an XCD's L2 holds the neighbour data its
+// rows gather instead of every XCD fetching the same rows.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = bid & 7;
+    return x * q + min(x, r) + (bid >> 3);
+}
+
 // The same partition with each XCD walking its range from the end.  The two
 // SpMV kernels of a BiCGSTAB iteration read the same matrix (~250 MB at 1M
 // rows, about the size of the 256 MB Infinity Cache); running the second

@@ -126,7 +126,7 @@ __global__ void k_bin_fixup(int nbins, const int* __restrict__ bstart, int* __re
 
 __global__ void k_gather_state(int n, const int* __restrict__ perm, StateView src, StateView dst,
                                const int* __restrict__ key, int* __restrict__ key_sorted) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     int o = perm[i];
     dst.pos[i] = src.pos[o];
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(BLOCK) k_neighbour(int n, GridInfo g, KernelCo
                                                      int* __restrict__ nbr, int* __restrict__ cnt,
                                                      double* __restrict__ L,
                                                      double* __restrict__ divr, uint32_t* err) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     int cx, cy;
     key_cell(g, key[i], cx, cy);
@@ -251,7 +251,7 @@ __global__ void k_effpor(int n, GridInfo g, KernelConsts kc, PorousParams pp, St
                          const int* __restrict__ key, const double2* __restrict__ spos,
                          const double* __restrict__ sm, const double* __restrict__ sr,
                          const int* __restrict__ sstart, uint32_t* err) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     s.R[i] = 0.0;
     if (s.tid[i] != 3) return;
@@ -324,7 +324,7 @@ __device__ __forceinline__ void load_L(const double* L, int i, double& l0, doubl
 __global__ void k_eddyvis(int n, KernelConsts kc, double dl, size_t ld, StateView s,
                           const int* __restrict__ nbr, const int* __restrict__ cnt,
                           const double* __restrict__ L) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n || s.tid[i] != 3) return;
     double l0, l1, l2, l3;
     load_L(L, i, l0, l1, l2, l3);
@@ -359,7 +359,7 @@ __global__ void k_eddyvis(int n, KernelConsts kc, double dl, size_t ld, StateVie
 __global__ void k_varts(int n, KernelConsts kc, double dl, size_t ld, StateView s,
                         const int* __restrict__ nbr, const int* __restrict__ cnt,
                         const double* __restrict__ L, const SimScalars* ss) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n || !(s.flags[i] & ISPH_FLAG_FREE)) return;
     double l0, l1, l2, l3;
     load_L(L, i, l0, l1, l2, l3);
@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(BLOCK) k_vstar(int n, KernelConsts kc, PorousP
                                                  const int* __restrict__ cnt,
                                                  const double* __restrict__ L,
                                                  const SimScalars* ss) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const int fi = s.flags[i];
     const double dt = ss->dt;
@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(BLOCK) k_assemble(int n, KernelConsts kc, doub
                                                     double* __restrict__ rsum,
                                                     int* __restrict__ dpos, double* __restrict__ rhs,
                                                     const SimScalars* ss, uint32_t* err) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const double dt = ss->dt;
     const int ti = s.tid[i];
@@ -579,7 +579,7 @@ __global__ void __launch_bounds__(BLOCK) k_comp_vel(int n, KernelConsts kc, size
                                                     const int* __restrict__ cnt,
                                                     const double* __restrict__ L,
                                                     const SimScalars* ss, uint32_t* err) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const int fi = s.flags[i];
     const double dt = ss->dt;
@@ -653,7 +653,7 @@ __global__ void k_comp_pos(int n, StateView s, const SimScalars* ss, double xrel
 // freesurf, isph.f90:49-86, pull form over the symmetric lists
 __global__ void k_freesurf(int n, double thr, size_t ld, StateView s, const int* __restrict__ nbr,
                            const int* __restrict__ cnt, const double* __restrict__ divr) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     bool v = false;
     const int c = cnt[i];
@@ -673,7 +673,7 @@ __global__ void __launch_bounds__(BLOCK) k_shift(int n, KernelConsts kc, double 
                                                  const int* __restrict__ nbr,
                                                  const int* __restrict__ cnt,
                                                  const double* __restrict__ L) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const int fi = s.flags[i];
     if (s.tid[i] != 3 || (fi & ISPH_FLAG_BUFFER)) return;

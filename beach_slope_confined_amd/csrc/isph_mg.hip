@@ -13,10 +13,11 @@
 //   level l+1  2x2 blocks of level-l cells, again Galerkin and 9-point;
 //   block     the levels from the first with at most MG_CMAX cells down to
 //            a single cell: V-cycles inside one block (k_mg_block).
-// Smoothing on the particle and grid levels is one damped-Jacobi sweep
-// before (from x = 0) and one after the coarse correction; the block levels
-// take MG_BLOCK_NU sweeps each way and MG_BLOCK_CYCLES V-cycles.  Every
-// coarse correction is scaled by MG_OVER.  The V-cycle is a fixed linear
+// Smoothing: the particle level takes one damped-Jacobi sweep after the
+// coarse correction and none before it (x = 0, the residual is b); the grid
+// levels one sweep before (from x = 0) and one after; the block levels
+// MG_BLOCK_NU sweeps each way, with max(2, ceil(K / 3)) V-cycles for K block
+// levels.  Every coarse correction is scaled by MG_OVER.  The V-cycle is a fixed linear
 // operator, so the preconditioned recurrences are the ordinary ones:
 //   p = r + beta (p - omega v),  p^ = M p,  v = A p^,  alpha = rho / (rhat, v)
 //   s = r - alpha v,             s^ = M s,  t = A s^,  omega = (t, s)/(t, t)
@@ -47,17 +48,19 @@ constexpr int MG_CMAX = 1024;  // top block level: at most this many cells (one 
 constexpr int MG_CLOW = 768;   // the block's lower levels together: at most this many cells
 constexpr int MG_CLEV = 16;    // at most this many block levels
 constexpr int MG_MAXLEV = 32;  // grid levels in all
-constexpr int MG_BLOCK_NU = 2;      // block levels: sweeps before and after the correction
-constexpr int MG_BLOCK_CYCLES = 2;  // block V-cycles per coarse solve
+constexpr int MG_BLOCK_NU = 3;      // block levels: sweeps before and after the correction
+// block V-cycles per coarse solve: a deeper block hierarchy compounds the
+// under-correction of its aggregates, so it gets more cycles
+inline int mg_block_cycles(int K) { return std::max(2, (K + 2) / 3); }
 // factor on every coarse-grid correction: the piecewise-constant
 // aggregates under-correct, and 1.5 cuts the BiCGSTAB iterations of the
 // 1024x1024 tank from 94 to 30 (1.3: 34; 1.7 in the block: 50)
 constexpr float MG_OVER = 1.5f;
-// Jacobi damping 0.7, rounded to the V-cycle's value type T
+// Jacobi damping 0.8, rounded to the V-cycle's value type T
 template <class T>
-__device__ __forceinline__ T mg_omega() { return (T)0.7; }
+__device__ __forceinline__ T mg_omega() { return (T)0.8; }
 template <>
-__device__ __forceinline__ float mg_omega<float>() { return 0.7f; }
+__device__ __forceinline__ float mg_omega<float>() { return 0.8f; }
 
 __device__ __forceinline__ int mg_col(int e, int) { return e; }
 __device__ __forceinline__ int mg_col(short e, int i) { return i + (int)e; }
@@ -133,40 +136,21 @@ __global__ void k_mg_galerkin(int nx, int ny, const T* __restrict__ A, int nx2, 
 // ---------------------------------------------------------------------------
 // Level 0 down.  The right-hand side b is formed from the outer pairs:
 // MODE 0: b = p = r + beta q (r on a restart), from rq = (r, q);
-// MODE 1: b = s = r - alpha v, from rv = (r, v).  b goes to bout (FP64,
-// the outer recurrences need it); x0 = omega b (Jacobi from 0, unit
-// diagonal) and r0 = b - A x0 = b - omega A b.
-template <class T, class TI, int MODE, bool FIRST>
-__global__ void __launch_bounds__(BLOCK) k_mg_fine_down(int n, size_t ld, const TI* __restrict__ nbr,
-                                                        const int* __restrict__ cnt,
-                                                        const T* __restrict__ val,
-                                                        const double2* __restrict__ pairs,
-                                                        const SolverScalars* sc,
-                                                        double* __restrict__ bout,
-                                                        T* __restrict__ x0,
-                                                        T* __restrict__ r0) {
+// MODE 1: b = s = r - alpha v, from rv = (r, v).  b goes to bout (FP64, the
+// outer recurrences need it).  Without pre-smoothing the particle iterate is
+// 0 and its residual is b itself, r0 = (T) b.
+template <class T, int MODE, bool FIRST>
+__global__ void k_mg_bpoint(int n, const double2* __restrict__ pairs, const SolverScalars* sc,
+                            double* __restrict__ bout, T* __restrict__ r0) {
     if (sc->done) return;
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const double coef = MODE == 0 ? sc->betas : sc->alpha;
-    auto bj = [&](int j) -> double {
-        const double2 u = pairs[j];
-        if (MODE == 0) return FIRST ? u.x : u.x + coef * u.y;
-        return u.x - coef * u.y;
-    };
-    const double bi = bj(i);
+    const double2 u = pairs[i];
+    const double bi = MODE == 0 ? (FIRST ? u.x : u.x + coef * u.y) : u.x - coef * u.y;
     bout[i] = bi;
-    const T fb = (T)bi;
-    T acc = fb;
-    const int c = cnt[i];
-    for (int k = 0; k < c; ++k) {
-        const size_t o = (size_t)k * ld + i;
-        acc += val[o] * (T)bj(mg_col(nbr[o], i));
-    }
-    x0[i] = mg_omega<T>() * fb;
-    r0[i] = fb - mg_omega<T>() * acc;
+    r0[i] = (T)bi;
 }
-
 // restriction to the cells: b1[c] = sum of r0 over the cell's particles
 template <class T>
 __global__ void k_mg_restrict1(GridInfo g, const int* __restrict__ bstart,
@@ -457,15 +441,6 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
     if (own) x[c] = xt[pt][c];
 }
 
-// coarse correction of level 0: x0 += over e1(cell)
-template <class T>
-__global__ void k_mg_prolong1(int n, const int* __restrict__ cell, const T* __restrict__ e1,
-                              float over, T* __restrict__ x0, const SolverScalars* sc) {
-    if (sc->done) return;
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) x0[i] += (T)over * e1[cell[i]];
-}
-
 // grid level up: z = x + over e(parent), out = z + omega D^-1 (b - A z)
 template <class T>
 __global__ void k_mg_grid_up(int nx, int ny, const T* __restrict__ A,
@@ -490,7 +465,17 @@ __global__ void k_mg_grid_up(int nx, int ny, const T* __restrict__ A,
     out[c] = dg != (T)0 ? zc + mg_omega<T>() * (b[c] - acc) / dg : (T)0;
 }
 
-// level 0 up: z = x0 + e1(cell) (k_mg_prolong1), out = z + omega (b - A z)
+// coarse correction of level 0: the particle iterate was 0, so it becomes
+// z = over e1(cell)
+template <class T>
+__global__ void k_mg_prolong1(int n, const int* __restrict__ cell, const T* __restrict__ e1,
+                              float over, T* __restrict__ z, const SolverScalars* sc) {
+    if (sc->done) return;
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) z[i] = (T)over * e1[cell[i]];
+}
+
+// level 0 up: one damped-Jacobi sweep from z, out = z + omega (b - A z)
 template <class T, class TI>
 __global__ void __launch_bounds__(BLOCK) k_mg_fine_up(int n, size_t ld, const TI* __restrict__ nbr,
                                                       const int* __restrict__ cnt,
@@ -500,7 +485,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_fine_up(int n, size_t ld, const TI
                                                       double* __restrict__ out,
                                                       const SolverScalars* sc) {
     if (sc->done) return;
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const T zi = z[i];
     T acc = zi;
@@ -646,8 +631,8 @@ struct MGData {
     int lb = 0;    // first level solved inside one block (k_mg_block)
     MGBlock blk;
     MGLevel lv[MG_MAXLEV];
-    double *x0 = nullptr, *r0 = nullptr;
-    int* cell = nullptr;
+    double* z0 = nullptr;  // particle residual r0 on the way down, iterate z on the way up
+    int* cell = nullptr;   // level-1 cell of every particle
 };
 
 template <class T>
@@ -669,7 +654,7 @@ void mg_free(isph_ctx* c) {
         for (auto q : p)
             if (q) (void)hipFree(q);
     }
-    void* p[] = {m->x0, m->r0, m->cell};
+    void* p[] = {m->z0, m->cell};
     for (auto q : p)
         if (q) (void)hipFree(q);
     delete m;
@@ -700,8 +685,7 @@ static int mg_alloc(isph_ctx* c) {
         ny = (ny + 1) / 2;
     }
     const size_t N = (size_t)c->cap;
-    rc |= dalloc(c, &m->x0, N);
-    rc |= dalloc(c, &m->r0, N);
+    rc |= dalloc(c, &m->z0, N);
     rc |= dalloc(c, &m->cell, N);
     if (rc) {
         mg_free(c);
@@ -721,7 +705,7 @@ static int mg_alloc(isph_ctx* c) {
     MGBlock& B = m->blk;
     B.K = m->nlev - m->lb;
     B.nu = MG_BLOCK_NU;
-    B.cycles = MG_BLOCK_CYCLES;
+    B.cycles = mg_block_cycles(B.K);
     B.over = MG_OVER;
     int off = 0;
     for (int k = 0; k < B.K; ++k) {
@@ -764,10 +748,12 @@ static void mg_vcycle(isph_ctx* c, const MGSystem& sys, const TI* nbr, const dou
     const int n = c->n;
     const size_t ld = (size_t)c->cap;
     const T* val = mg_vals<T>(sys);
-    k_mg_fine_down<T, TI, MODE, FIRST><<<nblk(n), BLOCK, 0, c->stream>>>(
-        n, ld, nbr, c->cnt, val, pairs, c->sc, bout, as<T>(m->x0), as<T>(m->r0));
+    // r0 shares z0's storage: the restriction has read it before the
+    // prolongation writes z0
+    k_mg_bpoint<T, MODE, FIRST><<<nblk(n), BLOCK, 0, c->stream>>>(n, pairs, c->sc, bout,
+                                                                   as<T>(m->z0));
     k_mg_restrict1<T><<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
-        g, c->bincnt, as<T>(m->r0), as<T>(m->lv[0].b), c->sc);
+        g, c->bincnt, as<T>(m->z0), as<T>(m->lv[0].b), c->sc);
     const int lb = m->lb;
     for (int l = 0; l < lb; ++l) {
         MGLevel &a = m->lv[l], &b = m->lv[l + 1];
@@ -784,9 +770,9 @@ static void mg_vcycle(isph_ctx* c, const MGSystem& sys, const TI* nbr, const dou
         std::swap(a.x, a.x2);
     }
     k_mg_prolong1<T><<<nblk(n), BLOCK, 0, c->stream>>>(n, m->cell, as<T>(m->lv[0].x), MG_OVER,
-                                                       as<T>(m->x0), c->sc);
+                                                       as<T>(m->z0), c->sc);
     k_mg_fine_up<T, TI><<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, val, bout,
-                                                           as<T>(m->x0), out, c->sc);
+                                                           as<T>(m->z0), out, c->sc);
 }
 
 template <class T, class TI>

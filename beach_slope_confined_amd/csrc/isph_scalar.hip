@@ -69,7 +69,7 @@ __global__ void k_scalart(int n, KernelConsts kh, SaltConsts sc, size_t ld, Stat
                           const double2* __restrict__ pos_nb, const int* __restrict__ nbr,
                           const int* __restrict__ cnt, const double4* __restrict__ coef,
                           double* __restrict__ cdiff) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n || !salt_live(s, i)) return;
     const double con1 = s.con[i];
     const double4 di = coef[i];
