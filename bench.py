@@ -697,6 +697,10 @@ def main():
         with open(pmc) as f:
             d = json.load(f)
         roofline["traffic"] = d.get(roofline["kernel"], {}).get("hbm_bytes_per_launch")
+        if roofline["traffic"]:
+            roofline["traffic_source"] = ("profiles/bicg_spmv_traffic.json: rocprofv3 --pmc, "
+                                          "read bytes by request size, WRITE_SIZE")
+            roofline["traffic_over_model"] = round(roofline["traffic"] / nbytes, 3)
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
