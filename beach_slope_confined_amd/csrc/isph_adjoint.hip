@@ -79,6 +79,13 @@ struct AdjointState {
     // the next transposed solve (the adjacent step, rows in another order)
     double* lam_pid = nullptr;
     int npid = 0;       // entries allocated in lam_pid
+    // every forward step's PPE solution (nsteps x rec_n doubles, row order):
+    // the backward's recomputation starts each solve from it, so the solve
+    // stops at its first true-residual test with the same bits; null when
+    // it did not fit in memory (the recomputation then solves in full)
+    double* rec = nullptr;
+    int64_t rec_cap = 0;  // doubles allocated in rec
+    int64_t rec_n = 0;    // particles of the recorded run (constant: no zones, no sink)
     int warm_npid = 0;  // entries the transposed solves read and write: npid in
                         // the backward sweep, 0 (zero start) otherwise
 };
@@ -556,6 +563,7 @@ void adj_free(isph_ctx* c) {
             if (q) (void)hipFree(q);
     }
     if (S->lam_pid) (void)hipFree(S->lam_pid);
+    if (S->rec) (void)hipFree(S->rec);
     delete S;
     c->adj = nullptr;
 }
@@ -581,10 +589,27 @@ int isph_adjoint_forward(isph_ctx* c, const isph_objective* obj, int32_t nsteps,
     HIPCHK(c, hipMemsetAsync(S->dsc, 0, sizeof(double) * (8 + 3 * OBJ_MAXP), c->stream));
     // porous closure and umax of the initial state (main.f90:35-37)
     if ((rc = run_effpor(c)) || (rc = run_timestep(c)) || (rc = sync_check(c))) return rc;
+    // room for every step's PPE solution; without it the backward solves
+    // its recomputation in full
+    const int64_t need = (int64_t)nsteps * std::max(c->n, 1);
+    if (S->rec_cap < need) {
+        if (S->rec) (void)hipFree(S->rec);
+        S->rec = nullptr;
+        S->rec_cap = 0;
+        if (hipMalloc((void**)&S->rec, sizeof(double) * (size_t)need) == hipSuccess)
+            S->rec_cap = need;
+        else
+            (void)hipGetLastError();  // out of memory is not an error here
+    }
+    S->rec_n = c->n;
+    const bool record = S->rec != nullptr;
     const double weight = 1.0 / nsteps;
     for (int k = 0; k < nsteps; ++k) {
         if (k % every == 0 && (rc = save_ckpt(c, S->seg[k / every]))) return rc;
-        if ((rc = run_step(c))) {
+        c->solve_record = record ? S->rec + (size_t)k * c->n : nullptr;
+        rc = run_step(c);
+        c->solve_record = nullptr;
+        if (rc) {
             prof_collect(c);
             return rc;
         }
@@ -644,7 +669,11 @@ int isph_adjoint_backward(isph_ctx* c, double* grad3, double* J, int64_t* adj_it
         if ((rc = load_ckpt(c, S->seg[g]))) return rc;
         for (int s = 0; s < m; ++s) {
             if ((rc = alloc_tape(c, S->tape[s]))) return rc;
-            if ((rc = tape_step(c, S->tape[s]))) return rc;
+            const bool replay = S->rec && c->n == S->rec_n;
+            c->solve_replay = replay ? S->rec + (size_t)(k0 + s) * c->n : nullptr;
+            rc = tape_step(c, S->tape[s]);
+            c->solve_replay = nullptr;
+            if (rc) return rc;
         }
         for (int s = m - 1; s >= 0; --s) {
             const StateView& end = s + 1 < m ? S->tape[s + 1].Sa

@@ -622,7 +622,8 @@ int run_solve(isph_ctx* c) {
         HIPCHK(c, hipMemcpyAsync(s.p, c->fsol, nd, hipMemcpyDeviceToDevice, c->stream));
         return ISPH_OK;
     }
-    HIPCHK(c, hipMemcpyAsync(x, s.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    const double* start = c->solve_replay ? c->solve_replay : s.p;
+    HIPCHK(c, hipMemcpyAsync(x, start, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     int rc = prec_mg(c->prm.precision) ? solve_mg(c)
              : mixed                                ? solve_mixed(c)
                                                     : solve_fp64(c);
@@ -634,6 +635,9 @@ int run_solve(isph_ctx* c) {
     if (rc != ISPH_OK) return rc;
     // pressure scatter (isph.f90:465-483): rows are particles
     HIPCHK(c, hipMemcpyAsync(s.p, x, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    if (c->solve_record)
+        HIPCHK(c, hipMemcpyAsync(c->solve_record, x, sizeof(double) * c->n,
+                                 hipMemcpyDeviceToDevice, c->stream));
     return ISPH_OK;
 }
 

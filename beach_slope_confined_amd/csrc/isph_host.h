@@ -123,6 +123,11 @@ struct isph_ctx {
     int64_t next_pid = 0;          // pid of the next inserted particle (int64: max pid + 1 may exceed int)
     double* probe_scratch = nullptr;  // this GPU's isph_probe_noalloc scratch (device address)
     isph_progress_fn progress = nullptr;  // isph_set_progress
+    // the adjoint's recorded PPE solutions: run_solve copies its solution to
+    // solve_record when set, and starts from solve_replay instead of the
+    // particles' pressures when set (device arrays of n doubles, row order)
+    double* solve_record = nullptr;
+    const double* solve_replay = nullptr;
     void* progress_user = nullptr;
     std::string msg;
 };
