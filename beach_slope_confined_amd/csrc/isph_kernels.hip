@@ -179,6 +179,13 @@ __global__ void __launch_bounds__(BLOCK) k_neighbour(int n, GridInfo g, KernelCo
                                                      int* __restrict__ nbr, int* __restrict__ cnt,
                                                      double* __restrict__ L,
                                                      double* __restrict__ divr, uint32_t* err) {
+    // the first NB_STAGE entries of each list are staged in LDS and stored
+    // after the scan, slot by slot, so that a wave's stores of slot k are one
+    // coalesced row segment; stored as found, the lanes' k-th neighbours
+    // arrive in different loop trips and each store is a scattered part of a
+    // line (546 MB of fabric writes for 84 MB of indices at 1M particles)
+    constexpr int NB_STAGE = 24;
+    __shared__ int snb[NB_STAGE][BLOCK];
     int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     int cx, cy;
@@ -196,7 +203,8 @@ __global__ void __launch_bounds__(BLOCK) k_neighbour(int n, GridInfo g, KernelCo
             double dx = pi.x - pj.x, dy = pi.y - pj.y;
             double r = sqrt(dx * dx + dy * dy);
             if (!(r < kc.twoh)) continue;
-            if (c < maxnbr) nbr[(size_t)c * ld + i] = j;
+            if (c < NB_STAGE) snb[c][threadIdx.x] = j;
+            else if (c < maxnbr) nbr[(size_t)c * ld + i] = j;
             c++;
             double wx, wy;
             kern_gradw(kc, r, dx, dy, wx, wy);
@@ -213,6 +221,8 @@ __global__ void __launch_bounds__(BLOCK) k_neighbour(int n, GridInfo g, KernelCo
         raise_err(err, ERR_CAPACITY);
         c = maxnbr;
     }
+    const int staged = c < NB_STAGE ? c : NB_STAGE;
+    for (int k = 0; k < staged; ++k) nbr[(size_t)k * ld + i] = snb[k][threadIdx.x];
     cnt[i] = c;
     int fl = s.flags[i] & ~ISPH_FLAG_FREE;
     double dv = 0.0;

@@ -162,6 +162,10 @@ def run_gpu(cfg, steps, warmup, precision, profile, tol=None, maxit=200000, list
 def step_errors(a, b):
     """max |p_b - p_a| / max |p_a| and the velocity error over the fluid,
     relative to the largest fluid speed of a"""
+    if not np.array_equal(a["pid"], b["pid"]):  # align by particle id
+        ia, ib = np.argsort(a["pid"]), np.argsort(b["pid"])
+        a = {k: v[ia] for k, v in a.items()}
+        b = {k: v[ib] for k, v in b.items()}
     f = a["tid"] == 3
     perr = float(np.abs(b["pressure"] - a["pressure"]).max() / np.abs(a["pressure"]).max())
     vmag = np.hypot(a["vx"][f], a["vy"][f]).max()
@@ -427,6 +431,37 @@ def model_mg_step_bytes(n, nnz_off, iters, val_bytes=4):
     k = nnz_off / n
     extra = (val_bytes - 4) * (6 * k + 2)
     return n * ((1284 + 60 * k) + iters * (36 * k + 364 + extra))
+
+
+def run_mixed_errors():
+    """Pressure and velocity error of the mixed-precision forms on the first
+    step of the small dam break (dx = 0.01) and of the adjoint configuration
+    (dx = 0.002), against a tightly solved FP64 step (1e-13) and against the
+    FP64 drop-in (1e-8), from the same input."""
+    from beach_slope_confined_amd import capi, configs
+    from beach_slope_confined_amd.params import (PREC_FP64, PREC_FP64_MG, PREC_MIXED_MG,
+                                                 PREC_MIXED_V64)
+    out = {}
+    for name, dx in (("dam_break_dx0.01", 0.01), ("dam_break_dx0.002", 0.002)):
+        cfg = configs.dam_break(dx)
+        q = {}
+        for key, prec, tol in (("tight", PREC_FP64, 1e-13), ("fp64", PREC_FP64, 1e-8),
+                               ("mixed_v64", PREC_MIXED_V64, 1e-8),
+                               ("mixed_mg", PREC_MIXED_MG, 1e-9), ("fp64_mg", PREC_FP64_MG, 1e-9)):
+            c = dict(cfg)
+            c["params"] = cfg["params"].copy(precision=prec, tol=tol, maxit=200000)
+            sim = configs.make_sim(c, capi.IsphSim)
+            sim.step(1)
+            q[key] = (sim.get_particles(), sim.scalars()["iters"])
+            sim.close()
+        r = {"n_total": len(cfg["parts"]["x"])}
+        for key in ("fp64", "mixed_v64", "mixed_mg", "fp64_mg"):
+            pe, ve = step_errors(q["tight"][0], q[key][0])
+            pf, vf = step_errors(q["fp64"][0], q[key][0])
+            r[key] = {"iters": q[key][1], "pressure_err_vs_tight": pe, "velocity_err_vs_tight": ve,
+                      "pressure_err_vs_fp64": pf, "velocity_err_vs_fp64": vf}
+        out[name] = r
+    return out
 
 
 def run_large(steps, with_adjoint):
@@ -848,6 +883,13 @@ def main():
             }
         except Exception as e:  # reported, not hidden
             line["fgmres_reference"] = {"error": str(e)}
+
+    if not args.no_mixed:
+        try:
+            log("mixed-precision errors on the small configurations")
+            line["mixed_errors"] = run_mixed_errors()
+        except Exception as e:  # reported, not hidden
+            line["mixed_errors"] = {"error": str(e)}
 
     if not args.no_mixed and not args.no_large:
         try:
