@@ -432,7 +432,7 @@ int adj_ppe(isph_ctx* c, const AdjView& a, const StateView& s, int* nbr, int* cn
             HIPCHK(c, hipMemsetAsync(c->bincnt, 0, sizeof(int) * (nb + 2), c->stream));
             k_bin_keys<<<nblk(n), BLOCK, 0, c->stream>>>(n, c->grid, s.pos, s.tid, s.flags, 0, 0.0,
                                                           0.0, nullptr, c->key_s, c->bincnt,
-                                                          c->rank, c->err);
+                                                          c->err);
             rc = scan(c, c->bincnt, nb + 2);
         }
         if (!rc) rc = take_errors(c);

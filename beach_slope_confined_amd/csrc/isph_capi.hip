@@ -135,8 +135,35 @@ int scan(isph_ctx* c, int* data, int64_t count) {
     return ISPH_OK;
 }
 
-// stable counting sort by bin key (deleted: key nbins); `first`: no
-// previous binning (isph_set_particles)
+// Stable LSD radix sort of the n keys c->key (values in [0, maxkey]): the
+// sorted keys into c->key_s, the permutation (sorted position -> input
+// position) into c->perm.  Passes alternate between those and c->rank /
+// c->rs_val so that the last one lands in key_s / perm.
+int radix_sort(isph_ctx* c, int n, int64_t maxkey) {
+    int bits = 1;
+    while (bits < 31 && (int64_t(1) << bits) <= maxkey) ++bits;
+    const int passes = (bits + RS_BITS - 1) / RS_BITS;
+    const int ntiles = (n + RS_TILE - 1) / RS_TILE;
+    const int* kin = c->key;
+    const int* vin = nullptr;
+    for (int p = 0; p < passes; ++p) {
+        const bool last_parity = ((passes - 1 - p) & 1) == 0;
+        int* kout = last_parity ? c->key_s : c->rank;
+        int* vout = last_parity ? c->perm : c->rs_val;
+        k_rs_hist<<<ntiles, BLOCK, 0, c->stream>>>(n, kin, p * RS_BITS, ntiles, c->rs_hist);
+        int rc = scan(c, c->rs_hist, (int64_t)RS_DIGITS * ntiles);
+        if (rc) return rc;
+        k_rs_scatter<<<ntiles, BLOCK, 0, c->stream>>>(n, kin, vin, p * RS_BITS, ntiles, c->rs_hist,
+                                                      kout, vout);
+        kin = kout;
+        vin = vout;
+    }
+    HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+// stable sort by bin key (deleted: key nbins); `first`: no previous binning
+// (isph_set_particles)
 int sort_particles(isph_ctx* c, bool with_sink, bool first) {
     const int n = c->n;
     const int64_t nb = c->grid.nbins;
@@ -146,13 +173,10 @@ int sort_particles(isph_ctx* c, bool with_sink, bool first) {
     if (n > 0)
         k_bin_keys<<<nblk(n), BLOCK, 0, c->stream>>>(
             n, c->grid, s.pos, s.tid, s.flags, with_sink && c->prm.sink, c->prm.xlcutoff,
-            c->prm.ytcutoff, first ? nullptr : c->key_s, c->key, c->bincnt, c->rank, c->err);
+            c->prm.ytcutoff, first ? nullptr : c->key_s, c->key, c->bincnt, c->err);
     int rc = scan(c, c->bincnt, nb + 2);
     if (rc) return rc;
-    if (n > 0) {
-        k_bin_scatter<<<nblk(n), BLOCK, 0, c->stream>>>(n, c->key, c->rank, c->bincnt, c->perm);
-        k_bin_fixup<<<nblk(nb + 1), BLOCK, 0, c->stream>>>((int)(nb + 1), c->bincnt, c->perm);
-    }
+    if (n > 0 && (rc = radix_sort(c, n, nb))) return rc;
     int nnew = n;
     if (with_sink && c->prm.sink) {
         HIPCHK(c, hipMemcpyAsync(&nnew, c->bincnt + nb, sizeof(int), hipMemcpyDeviceToHost,
@@ -160,7 +184,7 @@ int sort_particles(isph_ctx* c, bool with_sink, bool first) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     if (nnew > 0)
-        k_gather_state<<<nblk(nnew), BLOCK, 0, c->stream>>>(nnew, c->perm, s, d, c->key, c->key_s);
+        k_gather_state<<<nblk(nnew), BLOCK, 0, c->stream>>>(nnew, c->perm, s, d, nullptr, nullptr);
     HIPCHK(c, hipGetLastError());
     c->cur ^= 1;
     c->n = nnew;
@@ -739,10 +763,14 @@ int isph_create(const isph_params* prm, int32_t cap, int32_t cap_soil, isph_ctx*
     rc |= dalloc(c, &c->key_s, N);
     rc |= dalloc(c, &c->rank, N);
     rc |= dalloc(c, &c->perm, N);
+    rc |= dalloc(c, &c->rs_val, N);
+    rc |= dalloc(c, &c->rs_hist, (size_t)RS_DIGITS * ((N + RS_TILE - 1) / RS_TILE));
     rc |= dalloc(c, &c->bincnt, (size_t)nb + 2);
-    // block sums of the scans: over the bins of a re-binning and over the
-    // particles of a buffer removal
-    rc |= dalloc(c, &c->scanbuf, (size_t)nblk(std::max<int64_t>(nb + 2, (int64_t)N), SCAN_BLOCK) + 1);
+    // block sums of the scans: over the bins of a re-binning, over the tile
+    // digit counts of a radix pass and over the particles of a buffer removal
+    const int64_t scan_max = std::max<int64_t>(
+        std::max<int64_t>(nb + 2, (int64_t)N), (int64_t)RS_DIGITS * ((N + RS_TILE - 1) / RS_TILE));
+    rc |= dalloc(c, &c->scanbuf, (size_t)nblk(scan_max, SCAN_BLOCK) + 1);
     rc |= dalloc(c, &c->nbr, N * M);
     rc |= dalloc(c, &c->cnt, N);
     rc |= dalloc(c, &c->val64, N * M);
@@ -794,7 +822,8 @@ int isph_destroy(isph_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_state(c->st[0]);
     free_state(c->st[1]);
-    void* p[] = {c->L, c->divr, c->key, c->key_s, c->rank, c->perm, c->bincnt, c->scanbuf, c->nbr16,
+    void* p[] = {c->L, c->divr, c->key, c->key_s, c->rank, c->perm, c->rs_val, c->rs_hist,
+                 c->bincnt, c->scanbuf, c->nbr16,
                  c->rq64, c->rv64, c->rq32, c->rv32, c->rsum,
                  c->nbr, c->cnt, c->val64, c->val32, c->dpos, c->rhs, c->part, c->counters,
                  c->sc, c->ss, c->err, c->spos, c->smass, c->srho, c->sstart,

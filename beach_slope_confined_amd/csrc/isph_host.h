@@ -65,6 +65,7 @@ struct isph_ctx {
     double* L = nullptr;
     double* divr = nullptr;
     int *key = nullptr, *key_s = nullptr, *rank = nullptr, *perm = nullptr;
+    int *rs_hist = nullptr, *rs_val = nullptr;  // radix sort: tile digit counts, values
     int *bincnt = nullptr, *scanbuf = nullptr;
     int *nbr = nullptr, *cnt = nullptr;
     bool have_nbr = false;

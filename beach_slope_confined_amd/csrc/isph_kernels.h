@@ -10,6 +10,12 @@ namespace isph {
 constexpr int BLOCK = 256;       // threads per block (4 wave64)
 constexpr int SCAN_BLOCK = 1024;
 
+// LSD radix sort of the bin keys: 8-bit digits, tiles of 8 keys per thread
+constexpr int RS_BITS = 8;
+constexpr int RS_DIGITS = 1 << RS_BITS;  // == BLOCK: one digit per thread
+constexpr int RS_IPT = 8;
+constexpr int RS_TILE = BLOCK * RS_IPT;
+
 // device error word bits
 constexpr uint32_t ERR_DOMAIN = 1u;
 constexpr uint32_t ERR_CAPACITY = 2u;
@@ -285,12 +291,13 @@ constexpr int FIN_REPLACE = 7;  // (r,rhat) after a reliable update
 
 __global__ void k_bin_keys(int n, GridInfo g, const double2* pos, const int* tid, const int* flags,
                            int sink, double xlcut, double ytcut, const int* prev_key, int* key,
-                           int* bincnt, int* rank, uint32_t* err);
+                           int* bincnt, uint32_t* err);
 __global__ void k_scan_blocks(int n, int* data, int* bsum);
 __global__ void k_scan_sums(int nb, int* bsum);
 __global__ void k_scan_add(int n, int* data, const int* bsum);
-__global__ void k_bin_scatter(int n, const int* key, const int* rank, const int* bstart, int* perm);
-__global__ void k_bin_fixup(int nbins, const int* bstart, int* perm);
+__global__ void k_rs_hist(int n, const int* keys, int shift, int ntiles, int* hist);
+__global__ void k_rs_scatter(int n, const int* kin, const int* vin, int shift, int ntiles,
+                             const int* hist, int* kout, int* vout);
 __global__ void k_gather_state(int n, const int* perm, StateView src, StateView dst, const int* key,
                                int* key_sorted);
 __global__ void k_projection(int n, StateView s, const SimScalars* ss);

@@ -15,17 +15,19 @@ namespace isph {
 __device__ __forceinline__ void raise_err(uint32_t* err, uint32_t bit) { atomicOr(err, bit); }
 
 // ---------------------------------------------------------------------------
-// cell binning: stable counting sort by bin key (cellshift/cellshiftalt
-// re-binning, integrator.f90:32-391, + resetid, isph.f90:88-113).  With the
-// reference order a fluid particle moves at most one cell from its previous
-// bin (prev_key; null on the first binning), as cellshift only hands a
-// particle to the adjacent cell it crossed into (integrator.f90:271-310).
+// cell binning: stable sort by bin key (cellshift/cellshiftalt re-binning,
+// integrator.f90:32-391, + resetid, isph.f90:88-113).  k_bin_keys computes
+// each particle's key and the bin counts (their exclusive scan is bstart);
+// an LSD radix sort of (key, index) pairs gives the permutation, ties in the
+// previous order.  With the reference order a fluid particle moves at most
+// one cell from its previous bin (prev_key; null on the first binning), as
+// cellshift only hands a particle to the adjacent cell it crossed into
+// (integrator.f90:271-310).
 // ---------------------------------------------------------------------------
 __global__ void k_bin_keys(int n, GridInfo g, const double2* __restrict__ pos,
                            const int* __restrict__ tid, const int* __restrict__ flags, int sink,
                            double xlcut, double ytcut, const int* __restrict__ prev_key,
-                           int* __restrict__ key, int* __restrict__ bincnt,
-                           int* __restrict__ rank, uint32_t* err) {
+                           int* __restrict__ key, int* __restrict__ bincnt, uint32_t* err) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     double2 p = pos[i];
@@ -53,7 +55,18 @@ __global__ void k_bin_keys(int n, GridInfo g, const double2* __restrict__ pos,
         }
     }
     key[i] = k;
-    rank[i] = atomicAdd(&bincnt[k], 1);
+    // one atomic per distinct key of the wave: consecutive particles of a
+    // sorted state mostly share their bin, and atomics on one address
+    // serialise
+    unsigned long long todo = __ballot(1);
+    const int lane = threadIdx.x & 63;
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int lk = __shfl(k, leader, 64);
+        const unsigned long long same = __ballot(k == lk);
+        if (lane == leader) atomicAdd(&bincnt[lk], (int)__popcll(same));
+        todo &= ~same;
+    }
 }
 
 // exclusive scan of int32 in place: per-block scan, scan of block sums, add
@@ -101,29 +114,89 @@ __global__ void k_scan_add(int n, int* __restrict__ data, const int* __restrict_
     if (i < n) data[i] += bsum[blockIdx.x];
 }
 
-__global__ void k_bin_scatter(int n, const int* __restrict__ key, const int* __restrict__ rank,
-                              const int* __restrict__ bstart, int* __restrict__ perm) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    perm[bstart[key[i]] + rank[i]] = i;
+// LSD radix sort, one RS_BITS-bit digit per pass.  A tile of RS_TILE keys
+// is RS_IPT rounds of BLOCK consecutive keys; per pass, k_rs_hist counts
+// each tile's digits into hist[digit * ntiles + tile], whose exclusive scan
+// is the output offset of every (digit, tile), and k_rs_scatter writes each
+// key to its offset plus its rank among the equal digits before it in the
+// tile.  Every pass is stable, so keys end sorted with ties in input order.
+
+// lanes of the wave whose digit equals this lane's (among the `ok` lanes)
+__device__ __forceinline__ unsigned long long digit_peers(bool ok, int d) {
+    unsigned long long peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < RS_BITS; ++b) {
+        const unsigned long long m = __ballot((d >> b) & 1);
+        peers &= ((d >> b) & 1) ? m : ~m;
+    }
+    return peers;
 }
 
-// restore stability: within each bin, ascending previous position
-__global__ void k_bin_fixup(int nbins, const int* __restrict__ bstart, int* __restrict__ perm) {
-    int b = blockIdx.x * BLOCK + threadIdx.x;
-    if (b >= nbins) return;
-    int lo = bstart[b], hi = bstart[b + 1];
-    for (int a = lo + 1; a < hi; ++a) {
-        int v = perm[a];
-        int c = a - 1;
-        while (c >= lo && perm[c] > v) {
-            perm[c + 1] = perm[c];
-            --c;
+static_assert(RS_DIGITS == BLOCK, "one digit per thread");
+
+__global__ void __launch_bounds__(BLOCK) k_rs_hist(int n, const int* __restrict__ keys, int shift,
+                                                   int ntiles, int* __restrict__ hist) {
+    __shared__ int h[RS_DIGITS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const unsigned long long below = (1ull << (threadIdx.x & 63)) - 1;
+    const int base = blockIdx.x * RS_TILE;
+    for (int e = 0; e < RS_IPT; ++e) {
+        const int idx = base + e * BLOCK + threadIdx.x;
+        const bool ok = idx < n;
+        const int d = ok ? (keys[idx] >> shift) & (RS_DIGITS - 1) : 0;
+        const unsigned long long peers = digit_peers(ok, d);
+        if (ok && !(peers & below)) atomicAdd(&h[d], (int)__popcll(peers));
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// vin null: the values are the input positions
+__global__ void __launch_bounds__(BLOCK) k_rs_scatter(int n, const int* __restrict__ kin,
+                                                      const int* __restrict__ vin, int shift,
+                                                      int ntiles, const int* __restrict__ hist,
+                                                      int* __restrict__ kout,
+                                                      int* __restrict__ vout) {
+    constexpr int NW = BLOCK / 64;
+    __shared__ int base[RS_DIGITS];      // next offset of each digit in this tile
+    __shared__ int wcnt[2][NW][RS_DIGITS];  // per round (double-buffered): count per wave
+    const int wv = threadIdx.x >> 6;
+    const unsigned long long lower = (1ull << (threadIdx.x & 63)) - 1;
+    base[threadIdx.x] = hist[(size_t)threadIdx.x * ntiles + blockIdx.x];
+    for (int e = 0; e < RS_IPT; ++e) {
+        int(*wc)[RS_DIGITS] = wcnt[e & 1];
+        const int idx = blockIdx.x * RS_TILE + e * BLOCK + threadIdx.x;
+        const bool ok = idx < n;
+        int k = 0, v = 0;
+        if (ok) {
+            k = kin[idx];
+            v = vin ? vin[idx] : idx;
         }
-        perm[c + 1] = v;
+        const int d = (k >> shift) & (RS_DIGITS - 1);
+        for (int q = 0; q < NW; ++q) wc[q][threadIdx.x] = 0;
+        __syncthreads();  // zeroed counts; the previous round's base update done
+        const unsigned long long peers = digit_peers(ok, d);
+        const int rank = (int)__popcll(peers & lower);
+        if (ok && rank == 0) wc[wv][d] = (int)__popcll(peers);
+        __syncthreads();
+        if (ok) {
+            int pos = base[d] + rank;
+            for (int q = 0; q < wv; ++q) pos += wc[q][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();  // every read of base done before its update
+        int tot = 0;
+        for (int q = 0; q < NW; ++q) tot += wc[q][threadIdx.x];
+        base[threadIdx.x] += tot;
+        // wc is zeroed again two rounds on, after the next round's first
+        // barrier has ordered this update's reads
     }
 }
 
+// dst[i] = src[perm[i]]; key_sorted[i] = key[perm[i]] unless key_sorted is
+// null (the radix sort leaves the sorted keys in place)
 __global__ void k_gather_state(int n, const int* __restrict__ perm, StateView src, StateView dst,
                                const int* __restrict__ key, int* __restrict__ key_sorted) {
     int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
@@ -148,7 +221,7 @@ __global__ void k_gather_state(int n, const int* __restrict__ perm, StateView sr
     dst.tid[i] = src.tid[o];
     dst.flags[i] = src.flags[o];
     dst.pid[i] = src.pid[o];
-    key_sorted[i] = key[o];
+    if (key_sorted) key_sorted[i] = key[o];
 }
 
 // ---------------------------------------------------------------------------
@@ -172,6 +245,27 @@ __global__ void k_projection(int n, StateView s, const SimScalars* ss) {
 // neighbour + nsearch2, search.f90:13-399: list, KGC matrix A (:77-85),
 // divr (:272), L = inv(A) (:276-278), free flag (:302-304)
 // ---------------------------------------------------------------------------
+// Candidates of a particle in cell R (R = key / per, the cell's place in bin
+// order) are the particles of cells R + dc*ny + {-1, 0, 1}, dc in {-1, 0, 1}:
+// three contiguous index ranges, one per column.  A block's particles are
+// consecutive in bin order, so for each dc the ranges of all its particles
+// lie inside one window, [bstart[per*(R0+dc*ny-1)], bstart[per*(R1+dc*ny+2)))
+// for the block's first and last cells R0, R1.  The positions of a window
+// of up to NB_WIN particles are staged in LDS once for the block (2h cells
+// hold about 7 fluid particles, so a block's window is about 280).
+//
+// The scan runs in two phases.  Phase 1 keeps a candidate when its squared
+// distance d2 does not exceed twoh^2 (1 + 2^-30): every rejected candidate
+// has RN(sqrt(d2)) >= twoh (d2 > twoh^2 (1 + 2^-31) gives sqrt(d2) >
+// twoh (1 + 2^-33), and rounding is monotonic), so it fails the reference's
+// test r < 2h exactly as before.  Phase 2 walks the kept candidates in scan
+// order, applies the exact test to r = sqrt(d2) and accumulates A.  The lanes
+// of a wave reach the expensive part (sqrt, kernel gradient, two divisions)
+// together on every trip, where a one-phase scan pays it on nearly every
+// candidate trip (about one candidate in three falls inside 2h: the circle
+// covers 35% of the 3 x 3 cells of side 2h) because some lane of the wave is
+// nearly always on a hit.  List order, list
+// contents and the order of the sums are those of the one-phase scan.
 __global__ void __launch_bounds__(BLOCK) k_neighbour(int n, GridInfo g, KernelConsts kc,
                                                      double thr, int maxnbr, size_t ld,
                                                      StateView s, const int* __restrict__ key,
@@ -179,32 +273,72 @@ __global__ void __launch_bounds__(BLOCK) k_neighbour(int n, GridInfo g, KernelCo
                                                      int* __restrict__ nbr, int* __restrict__ cnt,
                                                      double* __restrict__ L,
                                                      double* __restrict__ divr, uint32_t* err) {
-    // the first NB_STAGE entries of each list are staged in LDS and stored
-    // after the scan, slot by slot, so that a wave's stores of slot k are one
-    // coalesced row segment; stored as found, the lanes' k-th neighbours
-    // arrive in different loop trips and each store is a scattered part of a
-    // line (546 MB of fabric writes for 84 MB of indices at 1M particles)
+    // the first NB_STAGE entries of each list are kept in LDS and stored after
+    // the scan, slot by slot, so that a wave's stores of slot k are one
+    // coalesced row segment (stored as found, the lanes' k-th entries arrive
+    // on different trips and each store is a scattered part of a line)
     constexpr int NB_STAGE = 24;
+    constexpr int NB_WIN = 320;  // staged candidate positions per column offset
     __shared__ int snb[NB_STAGE][BLOCK];
-    int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    __shared__ double2 swin[3][NB_WIN];
+    __shared__ int wlo[3], whi[3];
+    const int i0 = xcd_tile(blockIdx.x, gridDim.x) * BLOCK;
+    if (i0 >= n) return;  // whole block past the end: no barrier is pending
+    const int i = i0 + threadIdx.x;
+    if (threadIdx.x < 3) {
+        const int dc = (int)threadIdx.x - 1;
+        const int r0 = key[i0] / g.per, r1 = key[min(n, i0 + BLOCK) - 1] / g.per;
+        wlo[threadIdx.x] = bstart[g.per * (r0 + dc * g.ny - 1)];
+        whi[threadIdx.x] = bstart[g.per * (r1 + dc * g.ny + 2)];
+    }
+    __syncthreads();
+    for (int w = 0; w < 3; ++w) {
+        const int lo = wlo[w], len = whi[w] - lo;
+        if (len <= NB_WIN)
+            for (int k = threadIdx.x; k < len; k += BLOCK) swin[w][k] = s.pos[lo + k];
+    }
+    __syncthreads();
     if (i >= n) return;
-    int cx, cy;
-    key_cell(g, key[i], cx, cy);
+    const int R = key[i] / g.per;
     const double2 pi = s.pos[i];
+    const double t2 = (kc.twoh * kc.twoh) * (1.0 + 0x1p-30);
+    int lo[3], hi[3];
+    for (int w = 0; w < 3; ++w) {
+        const int cr = R + (w - 1) * g.ny;
+        lo[w] = bstart[g.per * (cr - 1)];
+        hi[w] = bstart[g.per * (cr + 2)];
+    }
+    // phase 1: candidates within the widened radius, in scan order.  A range
+    // outside its window (keys out of order, as after a buffer insertion
+    // that no re-binning has followed yet) is read from global memory.
+    int c1 = 0;
+    for (int w = 0; w < 3; ++w) {
+        const int wl = wlo[w];
+        const bool staged = whi[w] - wl <= NB_WIN && lo[w] >= wl && hi[w] <= whi[w];
+        for (int j = lo[w]; j < hi[w]; ++j) {
+            if (j == i) continue;
+            const double2 pj = staged ? swin[w][j - wl] : s.pos[j];
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y;
+            if (dx * dx + dy * dy > t2) continue;
+            if (c1 < NB_STAGE) snb[c1][threadIdx.x] = j;
+            else if (c1 < maxnbr) nbr[(size_t)c1 * ld + i] = j;
+            c1++;
+        }
+    }
     int c = 0;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (int dc = -1; dc <= 1; ++dc) {
-        int blo, bhi;
-        column_bins(g, cx + dc, cy - 1, cy + 1, blo, bhi);
-        const int lo = bstart[blo], hi = bstart[bhi];
-        for (int j = lo; j < hi; ++j) {
-            if (j == i) continue;
+    int staged_n;
+    if (c1 <= maxnbr) {
+        // phase 2: exact test and the sums of A, compacting the list in
+        // place (slot c <= slot k: no kept entry is overwritten unread)
+        for (int k = 0; k < c1; ++k) {
+            const int j = k < NB_STAGE ? snb[k][threadIdx.x] : nbr[(size_t)k * ld + i];
             const double2 pj = s.pos[j];
             double dx = pi.x - pj.x, dy = pi.y - pj.y;
             double r = sqrt(dx * dx + dy * dy);
             if (!(r < kc.twoh)) continue;
             if (c < NB_STAGE) snb[c][threadIdx.x] = j;
-            else if (c < maxnbr) nbr[(size_t)c * ld + i] = j;
+            else nbr[(size_t)c * ld + i] = j;
             c++;
             double wx, wy;
             kern_gradw(kc, r, dx, dy, wx, wy);
@@ -216,13 +350,38 @@ __global__ void __launch_bounds__(BLOCK) k_neighbour(int n, GridInfo g, KernelCo
             a2 = a2 + wx * res3;
             a3 = a3 + wy * res3;
         }
+        staged_n = c < NB_STAGE ? c : NB_STAGE;
+    } else {
+        // more candidates than list slots (phase 1 could not keep them all):
+        // the one-phase scan, entries straight to the list, every hit in the
+        // sums
+        for (int w = 0; w < 3; ++w) {
+            for (int j = lo[w]; j < hi[w]; ++j) {
+                if (j == i) continue;
+                const double2 pj = s.pos[j];
+                double dx = pi.x - pj.x, dy = pi.y - pj.y;
+                double r = sqrt(dx * dx + dy * dy);
+                if (!(r < kc.twoh)) continue;
+                if (c < maxnbr) nbr[(size_t)c * ld + i] = j;
+                c++;
+                double wx, wy;
+                kern_gradw(kc, r, dx, dy, wx, wy);
+                const double mj = s.mass[j], rj = s.rho[j];
+                double res1 = (mj * (pj.x - pi.x)) / rj;
+                double res3 = (mj * (pj.y - pi.y)) / rj;
+                a0 = a0 + wx * res1;
+                a1 = a1 + wy * res1;
+                a2 = a2 + wx * res3;
+                a3 = a3 + wy * res3;
+            }
+        }
+        if (c > maxnbr) {
+            raise_err(err, ERR_CAPACITY);
+            c = maxnbr;
+        }
+        staged_n = 0;
     }
-    if (c > maxnbr) {
-        raise_err(err, ERR_CAPACITY);
-        c = maxnbr;
-    }
-    const int staged = c < NB_STAGE ? c : NB_STAGE;
-    for (int k = 0; k < staged; ++k) nbr[(size_t)k * ld + i] = snb[k][threadIdx.x];
+    for (int k = 0; k < staged_n; ++k) nbr[(size_t)k * ld + i] = snb[k][threadIdx.x];
     cnt[i] = c;
     int fl = s.flags[i] & ~ISPH_FLAG_FREE;
     double dv = 0.0;

@@ -218,7 +218,10 @@ __global__ void k_mg_down(int nx, int ny, const T* __restrict__ A,
 // single cell, are solved inside one block of MG_CMAX threads by V-cycles:
 // thread c owns cell c of every block level.  The top block level keeps its
 // stencil in registers; the lower levels' stencils, all iterates and
-// right-hand sides sit in LDS, so a sweep costs one barrier.
+// right-hand sides sit in LDS, so a sweep costs one barrier.  The levels of
+// at most 64 cells run in the first wave alone, a sweep ending in a wave
+// synchronisation instead of a block barrier; the other waves wait at one
+// barrier until that wave has been down and up through them.
 struct MGBlock {
     int K;                        // block levels (1..MG_CLEV); 0 is the top, K-1 one cell
     int nu;                       // damped-Jacobi sweeps before and after the correction
@@ -300,7 +303,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
     };
     // lower-level sweep: from zero (x1 = omega D^-1 b into parity 1), or from
     // parity p, with the correction from level l+1 at parity q if corr
-    auto low_sweep = [&](int l, int p, bool zero, bool corr, int q) {
+    auto low_sweep = [&](int l, int p, bool zero, bool corr, int q, bool wave) {
         const int nx = B.nx[l], ny = B.ny[l], n = nx * ny, o = B.off[l];
         if (c < n) {
             const T* A = As + 9 * o;
@@ -339,11 +342,12 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
             }
             xl[zero ? 1 : p ^ 1][o + c] = nv;
         }
-        __syncthreads();
+        if (wave) __syncwarp();
+        else __syncthreads();
     };
     // residual of level l at parity p, summed over the children of each cell
     // of level l+1 into its right-hand side
-    auto low_restrict = [&](int l, int p) {
+    auto low_restrict = [&](int l, int p, bool wave) {
         const int nx = B.nx[l], ny = B.ny[l], n = nx * ny, o = B.off[l];
         const int ny2 = B.ny[l + 1];
         if (c < B.nx[l + 1] * ny2) {
@@ -366,7 +370,8 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
                 }
             bl[B.off[l + 1] + c] = s;
         }
-        __syncthreads();
+        if (wave) __syncwarp();
+        else __syncthreads();
     };
 
     if (K == 1) {  // a single cell: exact
@@ -374,6 +379,15 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
         return;
     }
     const int nu = B.nu;
+    // levels W.. (at most 64 cells each; the last level has one) run in the
+    // first wave alone
+    int W = K - 1;
+    for (int l = 1; l < K; ++l)
+        if (B.nx[l] * B.ny[l] <= 64) {
+            W = l;
+            break;
+        }
+    const bool w0 = c < 64;
     int pt = 0;  // parity of the top iterate
     for (int cyc = 0; cyc < B.cycles; ++cyc) {
         for (int s = 0; s < nu; ++s) {
@@ -401,36 +415,49 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
             __syncthreads();
         }
         // down the lower levels: smoothing from zero, restriction
-        for (int l = 1; l + 1 < K; ++l) {
-            low_sweep(l, 0, true, false, 0);
+        for (int l = 1; l < W; ++l) {
+            low_sweep(l, 0, true, false, 0, false);
             int p = 1;
             for (int s = 1; s < nu; ++s) {
-                low_sweep(l, p, false, false, 0);
+                low_sweep(l, p, false, false, 0, false);
                 p ^= 1;
             }
-            low_restrict(l, p);
+            low_restrict(l, p, false);
         }
-        // the single cell of the last level: exact
-        {
+        if (w0) {
+            for (int l = W; l + 1 < K; ++l) {
+                low_sweep(l, 0, true, false, 0, true);
+                int p = 1;
+                for (int s = 1; s < nu; ++s) {
+                    low_sweep(l, p, false, false, 0, true);
+                    p ^= 1;
+                }
+                low_restrict(l, p, true);
+            }
+            // the single cell of the last level: exact
             const int o = B.off[K - 1];
             if (c == 0) {
                 const T dg = As[9 * o + 4];
                 xl[1][o] = dg != (T)0 ? bl[o] / dg : (T)0;
             }
-            __syncthreads();
+            __syncwarp();
         }
-        // up: correction folded into the first post-smoothing sweep
+        // up: correction folded into the first post-smoothing sweep (every
+        // wave keeps the parities; the first wave sweeps levels W..)
         int q = 1;  // parity of level l+1's iterate
         for (int l = K - 2; l >= 1; --l) {
+            const bool wave = l >= W;
+            if (l == W - 1) __syncthreads();  // the first wave is done with levels W..
             int p = nu & 1;
-            low_sweep(l, p, false, true, q);
+            if (w0 || !wave) low_sweep(l, p, false, true, q, wave);
             p ^= 1;
             for (int s = 1; s < nu; ++s) {
-                low_sweep(l, p, false, false, 0);
+                if (w0 || !wave) low_sweep(l, p, false, false, 0, wave);
                 p ^= 1;
             }
             q = p;
         }
+        if (W == 1) __syncthreads();
         top_sweep(pt, true, q);
         pt ^= 1;
         for (int s = 1; s < nu; ++s) {
@@ -490,7 +517,22 @@ __global__ void __launch_bounds__(BLOCK) k_mg_fine_up(int n, size_t ld, const TI
     const T zi = z[i];
     T acc = zi;
     const int c = cnt[i];
-    for (int k = 0; k < c; ++k) {
+    int k = 0;
+    // four entries' offsets and values are loaded before their gathers, so
+    // that the gathers of a row are in flight together; the sum keeps the
+    // entry order
+    for (; k + 4 <= c; k += 4) {
+        const size_t o = (size_t)k * ld + i;
+        const int j0 = mg_col(nbr[o], i), j1 = mg_col(nbr[o + ld], i);
+        const int j2 = mg_col(nbr[o + 2 * ld], i), j3 = mg_col(nbr[o + 3 * ld], i);
+        const T v0 = val[o], v1 = val[o + ld], v2 = val[o + 2 * ld], v3 = val[o + 3 * ld];
+        const T z0 = z[j0], z1 = z[j1], z2 = z[j2], z3 = z[j3];
+        acc += v0 * z0;
+        acc += v1 * z1;
+        acc += v2 * z2;
+        acc += v3 * z3;
+    }
+    for (; k < c; ++k) {
         const size_t o = (size_t)k * ld + i;
         acc += val[o] * z[mg_col(nbr[o], i)];
     }
@@ -506,7 +548,21 @@ __device__ __forceinline__ double mg_spmv(int i, int c, double si, const TI* __r
                                           const double* __restrict__ x) {
     const double xi = x[i];
     double acc = 0.0;
-    for (int k = 0; k < c; ++k) {
+    int k = 0;
+    // loads of four entries ahead of their gathers, sum in entry order
+    for (; k + 4 <= c; k += 4) {
+        const size_t o = (size_t)k * ld + i;
+        const int j0 = mg_col(nbr[o], i), j1 = mg_col(nbr[o + ld], i);
+        const int j2 = mg_col(nbr[o + 2 * ld], i), j3 = mg_col(nbr[o + 3 * ld], i);
+        const double v0 = (double)val[o], v1 = (double)val[o + ld];
+        const double v2 = (double)val[o + 2 * ld], v3 = (double)val[o + 3 * ld];
+        const double x0 = x[j0], x1 = x[j1], x2 = x[j2], x3 = x[j3];
+        acc = acc + v0 * (x0 - xi);
+        acc = acc + v1 * (x1 - xi);
+        acc = acc + v2 * (x2 - xi);
+        acc = acc + v3 * (x3 - xi);
+    }
+    for (; k < c; ++k) {
         const size_t o = (size_t)k * ld + i;
         acc = acc + (double)val[o] * (x[mg_col(nbr[o], i)] - xi);
     }
