@@ -10,7 +10,8 @@ namespace isph {
 constexpr int BLOCK = 256;       // threads per block (4 wave64)
 constexpr int SCAN_BLOCK = 1024;
 
-// LSD radix sort of the bin keys: 8-bit digits, tiles of 8 keys per thread
+// LSD radix sort of the bin keys: 8-bit digits, tiles of RS_IPT keys per
+// thread
 constexpr int RS_BITS = 8;
 constexpr int RS_DIGITS = 1 << RS_BITS;  // == BLOCK: one digit per thread
 constexpr int RS_IPT = 8;

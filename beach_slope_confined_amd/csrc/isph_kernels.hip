@@ -120,6 +120,8 @@ __global__ void k_scan_add(int n, int* __restrict__ data, const int* __restrict_
 // is the output offset of every (digit, tile), and k_rs_scatter writes each
 // key to its offset plus its rank among the equal digits before it in the
 // tile.  Every pass is stable, so keys end sorted with ties in input order.
+// Both kernels load all RS_IPT rounds of their tile before the first use, so
+// a tile costs one memory latency, not one per round.
 
 // lanes of the wave whose digit equals this lane's (among the `ok` lanes)
 __device__ __forceinline__ unsigned long long digit_peers(bool ok, int d) {
@@ -137,14 +139,20 @@ static_assert(RS_DIGITS == BLOCK, "one digit per thread");
 __global__ void __launch_bounds__(BLOCK) k_rs_hist(int n, const int* __restrict__ keys, int shift,
                                                    int ntiles, int* __restrict__ hist) {
     __shared__ int h[RS_DIGITS];
+    const int base = blockIdx.x * RS_TILE + threadIdx.x;
+    int kk[RS_IPT];
+#pragma unroll
+    for (int e = 0; e < RS_IPT; ++e) {
+        const int idx = base + e * BLOCK;
+        kk[e] = idx < n ? keys[idx] : 0;
+    }
     h[threadIdx.x] = 0;
     __syncthreads();
     const unsigned long long below = (1ull << (threadIdx.x & 63)) - 1;
-    const int base = blockIdx.x * RS_TILE;
+#pragma unroll
     for (int e = 0; e < RS_IPT; ++e) {
-        const int idx = base + e * BLOCK + threadIdx.x;
-        const bool ok = idx < n;
-        const int d = ok ? (keys[idx] >> shift) & (RS_DIGITS - 1) : 0;
+        const bool ok = base + e * BLOCK < n;
+        const int d = (kk[e] >> shift) & (RS_DIGITS - 1);
         const unsigned long long peers = digit_peers(ok, d);
         if (ok && !(peers & below)) atomicAdd(&h[d], (int)__popcll(peers));
     }
@@ -159,20 +167,24 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter(int n, const int* __restri
                                                       int* __restrict__ kout,
                                                       int* __restrict__ vout) {
     constexpr int NW = BLOCK / 64;
-    __shared__ int base[RS_DIGITS];      // next offset of each digit in this tile
+    __shared__ int base[RS_DIGITS];         // next offset of each digit in this tile
     __shared__ int wcnt[2][NW][RS_DIGITS];  // per round (double-buffered): count per wave
     const int wv = threadIdx.x >> 6;
     const unsigned long long lower = (1ull << (threadIdx.x & 63)) - 1;
+    const int t0 = blockIdx.x * RS_TILE + threadIdx.x;
+    int kk[RS_IPT], vv[RS_IPT];
+#pragma unroll
+    for (int e = 0; e < RS_IPT; ++e) {
+        const int idx = t0 + e * BLOCK;
+        kk[e] = idx < n ? kin[idx] : 0;
+        vv[e] = idx < n ? (vin ? vin[idx] : idx) : 0;
+    }
     base[threadIdx.x] = hist[(size_t)threadIdx.x * ntiles + blockIdx.x];
+#pragma unroll
     for (int e = 0; e < RS_IPT; ++e) {
         int(*wc)[RS_DIGITS] = wcnt[e & 1];
-        const int idx = blockIdx.x * RS_TILE + e * BLOCK + threadIdx.x;
-        const bool ok = idx < n;
-        int k = 0, v = 0;
-        if (ok) {
-            k = kin[idx];
-            v = vin ? vin[idx] : idx;
-        }
+        const bool ok = t0 + e * BLOCK < n;
+        const int k = kk[e];
         const int d = (k >> shift) & (RS_DIGITS - 1);
         for (int q = 0; q < NW; ++q) wc[q][threadIdx.x] = 0;
         __syncthreads();  // zeroed counts; the previous round's base update done
@@ -184,7 +196,7 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter(int n, const int* __restri
             int pos = base[d] + rank;
             for (int q = 0; q < wv; ++q) pos += wc[q][d];
             kout[pos] = k;
-            vout[pos] = v;
+            vout[pos] = vv[e];
         }
         __syncthreads();  // every read of base done before its update
         int tot = 0;
@@ -673,9 +685,10 @@ __global__ void __launch_bounds__(BLOCK) k_assemble(int n, KernelConsts kc, doub
         double ddx = pj.x - pi.x, ddy = pj.y - pi.y;
         double rr = sqrt(ddx * ddx + ddy * ddy);
         double pdist = 1.0 / (rr * rr + lam);
-        double r, wx, wy;
-        pair_geom(kc, pi, pj, r, wx, wy);
+        // pair_geom's r is rr: (pi - pj)^2 and (pj - pi)^2 round alike
         const double dxp = pi.x - pj.x, dyp = pi.y - pj.y;
+        double wx, wy;
+        kern_gradw(kc, rr, dxp, dyp, wx, wy);
         double t1, t2;
         if (ti == 3) {
             const double rho_ij = (((2.0 * rhoj) * rhoi) * (1.0 / (nj * ni))) *

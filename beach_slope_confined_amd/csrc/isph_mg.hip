@@ -74,14 +74,63 @@ __global__ void k_mg_cells(int n, GridInfo g, const int* __restrict__ key, int* 
     cell[i] = cx * g.ny + cy;
 }
 
-// A1 = P^T A P: per cell, its rows' entries summed by the neighbour's cell
-// offset; the unit diagonal adds 1 per row.  An entry outside the 3x3 cells
-// raises ERR_DOMAIN (cannot happen for lists built from the cell stencil).
+// A1 = P^T A P in two passes.  k_mg_rows9: every row's entries summed by the
+// neighbour's cell offset (9 sums, the unit diagonal first), one thread per
+// row; an entry outside the 3x3 cells raises ERR_DOMAIN (cannot happen for
+// lists built from the cell stencil).  k_mg_galerkin1: per cell, its rows'
+// sums in row order.  A thread per cell walking its rows' entries serially
+// (about 140 at 2h cells) left the GPU nearly idle: 606 blocks at 1M.
+__global__ void __launch_bounds__(BLOCK) k_mg_rows9(int n, int ny, size_t ld,
+                                                    const int* __restrict__ nbr,
+                                                    const int* __restrict__ cnt,
+                                                    const double* __restrict__ val,
+                                                    const int* __restrict__ cell,
+                                                    double* __restrict__ part9,
+                                                    uint32_t* err) {
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int ci = cell[i];
+    const int cx = ci / ny, cy = ci - cx * ny;
+    double acc[9] = {0, 0, 0, 0, 1.0, 0, 0, 0, 0};
+    bool bad = false;
+    auto add = [&](int cj, double v) {
+        const int jx = cj / ny, jy = cj - jx * ny;
+        const int dx = jx - cx, dy = jy - cy;
+        if (dx < -1 || dx > 1 || dy < -1 || dy > 1) {
+            bad = true;
+            return;
+        }
+        const int dd = (dx + 1) * 3 + (dy + 1);
+#pragma unroll
+        for (int q = 0; q < 9; ++q)  // constant indices: acc stays in registers
+            if (q == dd) acc[q] += v;
+    };
+    const int c = cnt[i];
+    int k = 0;
+    // the loads of four entries are in flight together; the sums keep the
+    // entry order
+    for (; k + 4 <= c; k += 4) {
+        const size_t o = (size_t)k * ld + i;
+        const int j0 = nbr[o], j1 = nbr[o + ld], j2 = nbr[o + 2 * ld], j3 = nbr[o + 3 * ld];
+        const double v0 = val[o], v1 = val[o + ld], v2 = val[o + 2 * ld], v3 = val[o + 3 * ld];
+        const int c0 = cell[j0], c1 = cell[j1], c2 = cell[j2], c3 = cell[j3];
+        add(c0, v0);
+        add(c1, v1);
+        add(c2, v2);
+        add(c3, v3);
+    }
+    for (; k < c; ++k) {
+        const size_t o = (size_t)k * ld + i;
+        add(cell[nbr[o]], val[o]);
+    }
+    if (bad) atomicOr(err, ERR_DOMAIN);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) part9[(size_t)d * ld + i] = acc[d];
+}
+
 template <class T>
 __global__ void k_mg_galerkin1(GridInfo g, const int* __restrict__ bstart, size_t ld,
-                               const int* __restrict__ nbr, const int* __restrict__ cnt,
-                               const double* __restrict__ val, const int* __restrict__ cell,
-                               T* __restrict__ A1, uint32_t* err) {
+                               const double* __restrict__ part9, T* __restrict__ A1) {
     const int nc = g.nx * g.ny;
     int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= nc) return;
@@ -89,21 +138,10 @@ __global__ void k_mg_galerkin1(GridInfo g, const int* __restrict__ bstart, size_
     int lo, hi;
     column_bins(g, cx, cy, cy, lo, hi);
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = bstart[lo]; i < bstart[hi]; ++i) {
-        acc[4] += 1.0;
-        const int k1 = cnt[i];
-        for (int k = 0; k < k1; ++k) {
-            const size_t o = (size_t)k * ld + i;
-            const int cj = cell[nbr[o]];
-            const int jx = cj / g.ny, jy = cj - jx * g.ny;
-            const int dx = jx - cx, dy = jy - cy;
-            if (dx < -1 || dx > 1 || dy < -1 || dy > 1) {
-                atomicOr(err, ERR_DOMAIN);
-                continue;
-            }
-            acc[(dx + 1) * 3 + (dy + 1)] += val[o];
-        }
-    }
+    for (int i = bstart[lo]; i < bstart[hi]; ++i)
+#pragma unroll
+        for (int d = 0; d < 9; ++d) acc[d] += part9[(size_t)d * ld + i];
+#pragma unroll
     for (int d = 0; d < 9; ++d) A1[(size_t)d * nc + c] = (T)acc[d];
 }
 
@@ -218,10 +256,7 @@ __global__ void k_mg_down(int nx, int ny, const T* __restrict__ A,
 // single cell, are solved inside one block of MG_CMAX threads by V-cycles:
 // thread c owns cell c of every block level.  The top block level keeps its
 // stencil in registers; the lower levels' stencils, all iterates and
-// right-hand sides sit in LDS, so a sweep costs one barrier.  The levels of
-// at most 64 cells run in the first wave alone, a sweep ending in a wave
-// synchronisation instead of a block barrier; the other waves wait at one
-// barrier until that wave has been down and up through them.
+// right-hand sides sit in LDS, so a sweep costs one barrier.
 struct MGBlock {
     int K;                        // block levels (1..MG_CLEV); 0 is the top, K-1 one cell
     int nu;                       // damped-Jacobi sweeps before and after the correction
@@ -266,12 +301,31 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
         w = live ? omega / a[4] : (T)0;
         bc = b[c];
     }
-    for (int l = 1; l < K; ++l) {
-        const int n9 = 9 * B.nx[l] * B.ny[l];
-        const T* __restrict__ src = static_cast<const T*>(B.A[l]);
-        T* dst = As + 9 * B.off[l];
-        for (int e = c; e < n9; e += MG_CMAX) dst[e] = src[e];
+    // the lower levels' stencils, concatenated as in As (level l from
+    // 9 off[l]; the last level is one cell): every load is issued before the
+    // first store, so the copy costs one memory latency, not one per level
+    constexpr int NST = (9 * MG_CLOW + MG_CMAX - 1) / MG_CMAX;
+    const int tot9 = K > 1 ? 9 * (B.off[K - 1] + 1) : 0;
+    T st[NST];
+#pragma unroll
+    for (int r = 0; r < NST; ++r) {
+        const int e = c + r * MG_CMAX;
+        st[r] = (T)0;
+        if (e < tot9) {
+            const void* src = B.A[1];
+            int base = 0;
+#pragma unroll
+            for (int q = 2; q < MG_CLEV; ++q)  // constant indices into B
+                if (q < K && e >= 9 * B.off[q]) {
+                    src = B.A[q];
+                    base = 9 * B.off[q];
+                }
+            st[r] = static_cast<const T*>(src)[e - base];
+        }
     }
+#pragma unroll
+    for (int r = 0; r < NST; ++r)
+        if (c + r * MG_CMAX < tot9) As[c + r * MG_CMAX] = st[r];
     // LDS keeps what the last kernel on this CU left there: every iterate
     // starts from zero, and a sweep reads the coarser level's iterate only
     // when it applies its correction
@@ -303,7 +357,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
     };
     // lower-level sweep: from zero (x1 = omega D^-1 b into parity 1), or from
     // parity p, with the correction from level l+1 at parity q if corr
-    auto low_sweep = [&](int l, int p, bool zero, bool corr, int q, bool wave) {
+    auto low_sweep = [&](int l, int p, bool zero, bool corr, int q) {
         const int nx = B.nx[l], ny = B.ny[l], n = nx * ny, o = B.off[l];
         if (c < n) {
             const T* A = As + 9 * o;
@@ -342,12 +396,11 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
             }
             xl[zero ? 1 : p ^ 1][o + c] = nv;
         }
-        if (wave) __syncwarp();
-        else __syncthreads();
+        __syncthreads();
     };
     // residual of level l at parity p, summed over the children of each cell
     // of level l+1 into its right-hand side
-    auto low_restrict = [&](int l, int p, bool wave) {
+    auto low_restrict = [&](int l, int p) {
         const int nx = B.nx[l], ny = B.ny[l], n = nx * ny, o = B.off[l];
         const int ny2 = B.ny[l + 1];
         if (c < B.nx[l + 1] * ny2) {
@@ -370,8 +423,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
                 }
             bl[B.off[l + 1] + c] = s;
         }
-        if (wave) __syncwarp();
-        else __syncthreads();
+        __syncthreads();
     };
 
     if (K == 1) {  // a single cell: exact
@@ -379,15 +431,6 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
         return;
     }
     const int nu = B.nu;
-    // levels W.. (at most 64 cells each; the last level has one) run in the
-    // first wave alone
-    int W = K - 1;
-    for (int l = 1; l < K; ++l)
-        if (B.nx[l] * B.ny[l] <= 64) {
-            W = l;
-            break;
-        }
-    const bool w0 = c < 64;
     int pt = 0;  // parity of the top iterate
     for (int cyc = 0; cyc < B.cycles; ++cyc) {
         for (int s = 0; s < nu; ++s) {
@@ -415,49 +458,36 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
             __syncthreads();
         }
         // down the lower levels: smoothing from zero, restriction
-        for (int l = 1; l < W; ++l) {
-            low_sweep(l, 0, true, false, 0, false);
+        for (int l = 1; l + 1 < K; ++l) {
+            low_sweep(l, 0, true, false, 0);
             int p = 1;
             for (int s = 1; s < nu; ++s) {
-                low_sweep(l, p, false, false, 0, false);
+                low_sweep(l, p, false, false, 0);
                 p ^= 1;
             }
-            low_restrict(l, p, false);
+            low_restrict(l, p);
         }
-        if (w0) {
-            for (int l = W; l + 1 < K; ++l) {
-                low_sweep(l, 0, true, false, 0, true);
-                int p = 1;
-                for (int s = 1; s < nu; ++s) {
-                    low_sweep(l, p, false, false, 0, true);
-                    p ^= 1;
-                }
-                low_restrict(l, p, true);
-            }
-            // the single cell of the last level: exact
+        // the single cell of the last level: exact
+        {
             const int o = B.off[K - 1];
             if (c == 0) {
                 const T dg = As[9 * o + 4];
                 xl[1][o] = dg != (T)0 ? bl[o] / dg : (T)0;
             }
-            __syncwarp();
+            __syncthreads();
         }
-        // up: correction folded into the first post-smoothing sweep (every
-        // wave keeps the parities; the first wave sweeps levels W..)
+        // up: correction folded into the first post-smoothing sweep
         int q = 1;  // parity of level l+1's iterate
         for (int l = K - 2; l >= 1; --l) {
-            const bool wave = l >= W;
-            if (l == W - 1) __syncthreads();  // the first wave is done with levels W..
             int p = nu & 1;
-            if (w0 || !wave) low_sweep(l, p, false, true, q, wave);
+            low_sweep(l, p, false, true, q);
             p ^= 1;
             for (int s = 1; s < nu; ++s) {
-                if (w0 || !wave) low_sweep(l, p, false, false, 0, wave);
+                low_sweep(l, p, false, false, 0);
                 p ^= 1;
             }
             q = p;
         }
-        if (W == 1) __syncthreads();
         top_sweep(pt, true, q);
         pt ^= 1;
         for (int s = 1; s < nu; ++s) {
@@ -689,6 +719,7 @@ struct MGData {
     MGLevel lv[MG_MAXLEV];
     double* z0 = nullptr;  // particle residual r0 on the way down, iterate z on the way up
     int* cell = nullptr;   // level-1 cell of every particle
+    double* part9 = nullptr;  // per row, its entries summed by neighbour cell offset (9 x cap)
 };
 
 template <class T>
@@ -710,7 +741,7 @@ void mg_free(isph_ctx* c) {
         for (auto q : p)
             if (q) (void)hipFree(q);
     }
-    void* p[] = {m->z0, m->cell};
+    void* p[] = {m->z0, m->cell, m->part9};
     for (auto q : p)
         if (q) (void)hipFree(q);
     delete m;
@@ -743,6 +774,7 @@ static int mg_alloc(isph_ctx* c) {
     const size_t N = (size_t)c->cap;
     rc |= dalloc(c, &m->z0, N);
     rc |= dalloc(c, &m->cell, N);
+    rc |= dalloc(c, &m->part9, 9 * N);
     if (rc) {
         mg_free(c);
         return fail(c, ISPH_E_HIP, "multigrid hierarchy allocation failed");
@@ -783,8 +815,11 @@ static int mg_setup(isph_ctx* c, const MGSystem& sys) {
     MGData* m = c->mg;
     const GridInfo& g = c->grid;
     k_mg_cells<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, g, c->key_s, m->cell);
+    if (c->n)
+        k_mg_rows9<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, g.ny, (size_t)c->cap, c->nbr, c->cnt,
+                                                        sys.val64, m->cell, m->part9, c->err);
     k_mg_galerkin1<T><<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
-        g, c->bincnt, (size_t)c->cap, c->nbr, c->cnt, sys.val64, m->cell, as<T>(m->lv[0].A), c->err);
+        g, c->bincnt, (size_t)c->cap, m->part9, as<T>(m->lv[0].A));
     for (int l = 0; l + 1 < m->nlev; ++l) {
         const MGLevel &a = m->lv[l], &b = m->lv[l + 1];
         k_mg_galerkin<T><<<nblk((int64_t)b.nx * b.ny), BLOCK, 0, c->stream>>>(
