@@ -211,6 +211,14 @@ __global__ void k_mg_restrict1(GridInfo g, const int* __restrict__ bstart,
 // them), so a clamped neighbour adds 0 and the sweeps need no branches
 __device__ __forceinline__ int mg_clamp(int e, int n) { return min(max(e, 0), n - 1); }
 
+// c / ny for a block cell, c < MG_CMAX and ny <= MG_CMAX, from rny = RN(1/ny)
+// in FP32: with c = k ny + r, (c + 1/2) / ny lies at least 1/(2 ny) >=
+// 2^-11 inside (k, k + 1), and the two FP32 roundings move it by at most
+// 2^-22 (c + 1/2) / ny < 2^-12, so the truncation is k.
+__device__ __forceinline__ int mg_div(int c, float rny) {
+    return (int)(((float)c + 0.5f) * rny);
+}
+
 template <class T>
 __device__ __forceinline__ T mg_jac0(const T* __restrict__ A, int nc, int c,
                                      const T* __restrict__ b) {
@@ -264,12 +272,104 @@ struct MGBlock {
     float over;                   // factor on the coarse correction
     int nx[MG_CLEV], ny[MG_CLEV];
     int off[MG_CLEV];             // offset of level l >= 1 in the lower-level arrays
+    float rny[MG_CLEV];           // 1 / ny in FP32 (mg_div)
     const void* A[MG_CLEV];       // stencils, of the V-cycle's value type
+    int D;                        // level solved exactly by its inverse (0: none)
 };
+
+// The exact coarse solve.  The first block level below the top with at most
+// MG_DMAX cells (and more than one) is solved by its inverse, formed once per
+// hierarchy by k_mg_coarse_inv: Gauss-Jordan with partial pivoting in FP64 on
+// the dense matrix of the level's 9-point stencil.  A cell with a zero
+// diagonal (an empty aggregate, which no other cell couples to) gets an
+// identity row, so its value stays 0 as in the sweeps.  If a pivot falls
+// below 1e-12 of the largest entry, `ok` is 0 and the V-cycles recurse to
+// the single cell instead.  The inverse is stored transposed, Ainv[j n + c].
+constexpr int MG_DMAX = 64;
+
+template <class T>
+__global__ void __launch_bounds__(MG_CMAX) k_mg_coarse_inv(int nx, int ny, const T* __restrict__ A,
+                                                           T* __restrict__ Ainv,
+                                                           int* __restrict__ ok) {
+    constexpr int W = 2 * MG_DMAX;  // [M | I]
+    __shared__ double M[MG_DMAX][W + 1];
+    __shared__ double fac[MG_DMAX];
+    __shared__ double amax;
+    __shared__ int piv, good;
+    const int n = nx * ny, t = threadIdx.x;
+    for (int e = t; e < MG_DMAX * W; e += MG_CMAX) M[e / W][e % W] = 0.0;
+    __syncthreads();
+    if (t < n) {
+        const int rx = t / ny, ry = t - rx * ny;
+        for (int d = 0; d < 9; ++d) {
+            const int ex = rx + d / 3 - 1, ey = ry + d % 3 - 1;
+            if (ex >= 0 && ex < nx && ey >= 0 && ey < ny)
+                M[t][ex * ny + ey] = (double)A[(size_t)d * n + t];
+        }
+        if (M[t][t] == 0.0) M[t][t] = 1.0;  // empty aggregate: x = b = 0
+        M[t][MG_DMAX + t] = 1.0;
+    }
+    __syncthreads();
+    if (t < 64) {  // largest |entry|, the scale of the pivot test
+        double v = 0.0;
+        if (t < n)
+            for (int j = 0; j < n; ++j) v = fmax(v, fabs(M[t][j]));
+        for (int s = 1; s < 64; s <<= 1) v = fmax(v, __shfl_xor(v, s, 64));
+        if (t == 0) {
+            amax = v;
+            good = 1;
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        if (t < 64) {  // pivot: largest |M[r][k]| over r >= k, lowest r on ties
+            double v = (t >= k && t < n) ? fabs(M[t][k]) : -1.0;
+            int r = t;
+            for (int s = 1; s < 64; s <<= 1) {
+                const double v2 = __shfl_xor(v, s, 64);
+                const int r2 = __shfl_xor(r, s, 64);
+                if (v2 > v || (v2 == v && r2 < r)) {
+                    v = v2;
+                    r = r2;
+                }
+            }
+            if (t == 0) {
+                piv = r;
+                if (!(v > 1e-12 * amax)) good = 0;
+            }
+        }
+        __syncthreads();
+        if (!good) break;
+        const int p = piv;
+        if (p != k && t < W) {
+            const double u = M[k][t];
+            M[k][t] = M[p][t];
+            M[p][t] = u;
+        }
+        __syncthreads();
+        const double pv = M[k][k];
+        if (t < n) fac[t] = M[t][k];
+        __syncthreads();
+        if (t < W) M[k][t] = M[k][t] / pv;
+        __syncthreads();
+        for (int e = t; e < n * W; e += MG_CMAX) {
+            const int r = e / W, j = e - r * W;
+            if (r != k) M[r][j] = M[r][j] - fac[r] * M[k][j];
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < n * n; e += MG_CMAX) {
+        const int j = e / n, c = e - j * n;
+        Ainv[e] = (T)M[c][MG_DMAX + j];
+    }
+    if (t == 0) *ok = good;
+}
 
 template <class T>
 __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __restrict__ b,
                                                       T* __restrict__ x,
+                                                      const T* __restrict__ Ainv,
+                                                      const int* __restrict__ Aok,
                                                       const SolverScalars* sc) {
     if (sc->done) return;
     __shared__ T xt[2][MG_CMAX];   // top level: iterate (two parities), residual
@@ -277,6 +377,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
     __shared__ T As[9 * MG_CLOW];  // lower levels: stencil d of cell c at 9 off + d n + c
     __shared__ T xl[2][MG_CLOW];
     __shared__ T bl[MG_CLOW];
+    __shared__ T Ai[MG_DMAX * MG_DMAX];  // the exact level's inverse, transposed
     const T omega = mg_omega<T>();
     const int c = threadIdx.x;
     const int K = B.K;
@@ -326,6 +427,10 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
 #pragma unroll
     for (int r = 0; r < NST; ++r)
         if (c + r * MG_CMAX < tot9) As[c + r * MG_CMAX] = st[r];
+    // the exact level (uniform: every thread reads the same flag)
+    const int D = (B.D > 0 && *Aok) ? B.D : 0;
+    const int nD = D ? B.nx[D] * B.ny[D] : 0;
+    for (int e = c; e < nD * nD; e += MG_CMAX) Ai[e] = Ainv[e];
     // LDS keeps what the last kernel on this CU left there: every iterate
     // starts from zero, and a sweep reads the coarser level's iterate only
     // when it applies its correction
@@ -357,44 +462,47 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
     };
     // lower-level sweep: from zero (x1 = omega D^-1 b into parity 1), or from
     // parity p, with the correction from level l+1 at parity q if corr
+    // Every LDS load of a sweep is issued before its first use: the diagonal
+    // selects the result at the end instead of guarding the loads, and the
+    // cell's column comes from B.rny (see mg_div) instead of an integer
+    // division; the arithmetic is unchanged.
     auto low_sweep = [&](int l, int p, bool zero, bool corr, int q) {
         const int nx = B.nx[l], ny = B.ny[l], n = nx * ny, o = B.off[l];
         if (c < n) {
             const T* A = As + 9 * o;
             const T dg = A[4 * n + c];
-            T nv = (T)0;
-            if (dg != (T)0) {
-                if (zero) {
-                    nv = omega * bl[o + c] / dg;
-                } else {
-                    const int cx = c / ny, cy = c - cx * ny;
-                    const T* xc = xl[p] + o;
-                    T acc = (T)0;
-                    T zi;
-                    if (corr) {
-                        const int ny2 = B.ny[l + 1];
-                        const T* e = xl[q] + B.off[l + 1];
-                        const T f = (T)B.over;
+            const T bc = bl[o + c];
+            T nv;
+            if (zero) {
+                nv = omega * bc / dg;
+            } else {
+                const int cx = mg_div(c, B.rny[l]), cy = c - cx * ny;
+                const T* xc = xl[p] + o;
+                T acc = (T)0;
+                T zi;
+                if (corr) {
+                    const int ny2 = B.ny[l + 1];
+                    const T* e = xl[q] + B.off[l + 1];
+                    const T f = (T)B.over;
 #pragma unroll
-                        for (int d = 0; d < 9; ++d) {
-                            const int ex = mg_clamp(cx + d / 3 - 1, nx);
-                            const int ey = mg_clamp(cy + d % 3 - 1, ny);
-                            acc += A[d * n + c] * (xc[ex * ny + ey] + f * e[(ex >> 1) * ny2 + (ey >> 1)]);
-                        }
-                        zi = xc[c] + f * e[(cx >> 1) * ny2 + (cy >> 1)];
-                    } else {
-#pragma unroll
-                        for (int d = 0; d < 9; ++d) {
-                            const int ex = mg_clamp(cx + d / 3 - 1, nx);
-                            const int ey = mg_clamp(cy + d % 3 - 1, ny);
-                            acc += A[d * n + c] * xc[ex * ny + ey];
-                        }
-                        zi = xc[c];
+                    for (int d = 0; d < 9; ++d) {
+                        const int ex = mg_clamp(cx + d / 3 - 1, nx);
+                        const int ey = mg_clamp(cy + d % 3 - 1, ny);
+                        acc += A[d * n + c] * (xc[ex * ny + ey] + f * e[(ex >> 1) * ny2 + (ey >> 1)]);
                     }
-                    nv = zi + omega * (bl[o + c] - acc) / dg;
+                    zi = xc[c] + f * e[(cx >> 1) * ny2 + (cy >> 1)];
+                } else {
+#pragma unroll
+                    for (int d = 0; d < 9; ++d) {
+                        const int ex = mg_clamp(cx + d / 3 - 1, nx);
+                        const int ey = mg_clamp(cy + d % 3 - 1, ny);
+                        acc += A[d * n + c] * xc[ex * ny + ey];
+                    }
+                    zi = xc[c];
                 }
+                nv = zi + omega * (bc - acc) / dg;
             }
-            xl[zero ? 1 : p ^ 1][o + c] = nv;
+            xl[zero ? 1 : p ^ 1][o + c] = dg != (T)0 ? nv : (T)0;
         }
         __syncthreads();
     };
@@ -404,7 +512,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
         const int nx = B.nx[l], ny = B.ny[l], n = nx * ny, o = B.off[l];
         const int ny2 = B.ny[l + 1];
         if (c < B.nx[l + 1] * ny2) {
-            const int X = c / ny2, Y = c - X * ny2;
+            const int X = mg_div(c, B.rny[l + 1]), Y = c - X * ny2;
             const T* A = As + 9 * o;
             const T* xc = xl[p] + o;
             T s = (T)0;
@@ -458,7 +566,8 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
             __syncthreads();
         }
         // down the lower levels: smoothing from zero, restriction
-        for (int l = 1; l + 1 < K; ++l) {
+        const int Lx = D ? D : K - 1;  // the level solved exactly
+        for (int l = 1; l < Lx; ++l) {
             low_sweep(l, 0, true, false, 0);
             int p = 1;
             for (int s = 1; s < nu; ++s) {
@@ -467,8 +576,15 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
             }
             low_restrict(l, p);
         }
-        // the single cell of the last level: exact
-        {
+        if (D) {  // x = Ainv b on level D
+            const int o = B.off[D];
+            if (c < nD) {
+                T acc = (T)0;
+                for (int j = 0; j < nD; ++j) acc += Ai[j * nD + c] * bl[o + j];
+                xl[1][o + c] = acc;
+            }
+            __syncthreads();
+        } else {  // the single cell of the last level
             const int o = B.off[K - 1];
             if (c == 0) {
                 const T dg = As[9 * o + 4];
@@ -478,7 +594,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
         }
         // up: correction folded into the first post-smoothing sweep
         int q = 1;  // parity of level l+1's iterate
-        for (int l = K - 2; l >= 1; --l) {
+        for (int l = Lx - 1; l >= 1; --l) {
             int p = nu & 1;
             low_sweep(l, p, false, true, q);
             p ^= 1;
@@ -720,6 +836,8 @@ struct MGData {
     double* z0 = nullptr;  // particle residual r0 on the way down, iterate z on the way up
     int* cell = nullptr;   // level-1 cell of every particle
     double* part9 = nullptr;  // per row, its entries summed by neighbour cell offset (9 x cap)
+    double* ainv = nullptr;   // inverse of the exact block level (MG_DMAX^2 values)
+    int* aok = nullptr;       // 1 when ainv holds it
 };
 
 template <class T>
@@ -741,7 +859,7 @@ void mg_free(isph_ctx* c) {
         for (auto q : p)
             if (q) (void)hipFree(q);
     }
-    void* p[] = {m->z0, m->cell, m->part9};
+    void* p[] = {m->z0, m->cell, m->part9, m->ainv, m->aok};
     for (auto q : p)
         if (q) (void)hipFree(q);
     delete m;
@@ -775,6 +893,8 @@ static int mg_alloc(isph_ctx* c) {
     rc |= dalloc(c, &m->z0, N);
     rc |= dalloc(c, &m->cell, N);
     rc |= dalloc(c, &m->part9, 9 * N);
+    rc |= dalloc(c, &m->ainv, (size_t)MG_DMAX * MG_DMAX);
+    rc |= dalloc(c, &m->aok, 1);
     if (rc) {
         mg_free(c);
         return fail(c, ISPH_E_HIP, "multigrid hierarchy allocation failed");
@@ -801,9 +921,17 @@ static int mg_alloc(isph_ctx* c) {
         B.nx[k] = L.nx;
         B.ny[k] = L.ny;
         B.A[k] = L.A;
+        B.rny[k] = 1.0f / (float)L.ny;
         B.off[k] = k == 0 ? 0 : off;
         if (k > 0) off += L.nx * L.ny;
     }
+    // the exact level: the first below the top with 2..MG_DMAX cells
+    B.D = 0;
+    for (int k = 1; k + 1 < B.K; ++k)
+        if (B.nx[k] * B.ny[k] <= MG_DMAX) {
+            B.D = k;
+            break;
+        }
     return ISPH_OK;
 }
 
@@ -824,6 +952,11 @@ static int mg_setup(isph_ctx* c, const MGSystem& sys) {
         const MGLevel &a = m->lv[l], &b = m->lv[l + 1];
         k_mg_galerkin<T><<<nblk((int64_t)b.nx * b.ny), BLOCK, 0, c->stream>>>(
             a.nx, a.ny, as<T>(a.A), b.nx, b.ny, as<T>(b.A));
+    }
+    if (m->blk.D) {
+        const MGLevel& L = m->lv[m->lb + m->blk.D];
+        k_mg_coarse_inv<T><<<1, MG_CMAX, 0, c->stream>>>(L.nx, L.ny, as<T>(L.A), as<T>(m->ainv),
+                                                          m->aok);
     }
     HIPCHK(c, hipGetLastError());
     return ISPH_OK;
@@ -852,7 +985,7 @@ static void mg_vcycle(isph_ctx* c, const MGSystem& sys, const TI* nbr, const dou
             a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.nx, b.ny, as<T>(b.b), c->sc);
     }
     k_mg_block<T><<<1, MG_CMAX, 0, c->stream>>>(m->blk, as<T>(m->lv[lb].b), as<T>(m->lv[lb].x),
-                                                 c->sc);
+                                                 as<T>(m->ainv), m->aok, c->sc);
     for (int l = lb - 1; l >= 0; --l) {
         MGLevel &a = m->lv[l], &b = m->lv[l + 1];
         k_mg_grid_up<T><<<nblk((int64_t)a.nx * a.ny), BLOCK, 0, c->stream>>>(

@@ -807,6 +807,15 @@ def main():
                 "iters_per_step": rmg["iters"][args.warmup:], "reliable_updates": rmg["outer"],
                 "relres": rmg["relres"], "pressure_err_vs_fp64": pe, "velocity_err_vs_fp64": ve,
             }
+            # whole-step byte model of the multigrid step (as for the large beach)
+            sb = model_mg_step_bytes(r64["n"], r64["nnz_off"],
+                                     float(np.mean(rmg["iters"][args.warmup:])))
+            ach = sb / (rmg["seconds"] / args.steps) / 1e9
+            line["mixed_mg"]["step_model"] = {
+                "bytes_per_step": round(sb), "achieved": round(ach, 1), "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "model": "rows x (1284 + 60K + iterations x (36K + 364)) bytes, K = %.3f"
+                         % (r64["nnz_off"] / r64["n"])}
         except Exception as e:  # reported, not hidden
             line["mixed_mg"] = {"error": str(e)}
         if "value" in line["mixed_mg"]:
