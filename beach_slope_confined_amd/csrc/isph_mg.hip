@@ -638,6 +638,215 @@ __global__ void k_mg_grid_up(int nx, int ny, const T* __restrict__ A,
     out[c] = dg != (T)0 ? zc + mg_omega<T>() * (b[c] - acc) / dg : (T)0;
 }
 
+// Two grid levels in one launch.  Each dependent launch of a cell level
+// costs a few microseconds whatever its size, so consecutive levels are
+// paired: one block takes a tile of MG_TX x MG_TY cells of the coarser of
+// the pair's outputs and recomputes the one-cell halos it needs on the
+// finer levels.  Every value is formed by the same expression, in the same
+// order, with the same clamped neighbours as k_mg_down / k_mg_grid_up, so
+// the results are theirs bit for bit; each cell is written by the one tile
+// that owns it.
+constexpr int MG_TX = 4, MG_TY = 8;
+
+// k_mg_down for levels 0 and 1 of the pair: level 0 (nx0 x ny0) smoothed
+// from zero and restricted into level 1, level 1 likewise into level 2.
+// The tile owns the level-2 cells [X0, X0+TX) x [Y0, Y0+TY), their level-1
+// children and level-0 grandchildren.
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_mg_down2(int nx0, int ny0, const T* __restrict__ A0,
+                                                    const T* __restrict__ b0, T* __restrict__ x0,
+                                                    int nx1, int ny1, const T* __restrict__ A1,
+                                                    T* __restrict__ b1, T* __restrict__ x1,
+                                                    int nx2, int ny2, T* __restrict__ b2,
+                                                    const SolverScalars* sc) {
+    if (sc->done) return;
+    constexpr int TX = MG_TX, TY = MG_TY;
+    constexpr int MW = 2 * TX + 2, MH = 2 * TY + 2;  // level 1 under the tile, halo 1
+    constexpr int RW = 2 * MW, RH = 2 * MH;          // their level-0 children
+    constexpr int FW = RW + 2, FH = RH + 2;          // ... with a halo of 1
+    __shared__ T xf[FW * FH];
+    __shared__ T rf[RW * RH];
+    __shared__ T bm[MW * MH];
+    __shared__ T xm[MW * MH];
+    __shared__ T rm[4 * TX * TY];
+    const int nty = (ny2 + TY - 1) / TY;
+    const int X0 = (blockIdx.x / nty) * TX, Y0 = (blockIdx.x % nty) * TY;
+    const int mx0 = 2 * X0 - 1, my0 = 2 * Y0 - 1;  // origin of the level-1 region
+    const int rx0 = 2 * mx0, ry0 = 2 * my0;        // origin of its children
+    const int fx0 = rx0 - 1, fy0 = ry0 - 1;        // origin of the level-0 region
+    const int nc0 = nx0 * ny0, nc1 = nx1 * ny1;
+    const T omega = mg_omega<T>();
+    // level 0 from zero, x = omega D^-1 b (each position holds its clamped
+    // cell's value, as the sweeps read clamped neighbours)
+    for (int e = threadIdx.x; e < FW * FH; e += BLOCK) {
+        const int i = e / FH, j = e - i * FH;
+        const int ox = fx0 + i, oy = fy0 + j;
+        const int c = mg_clamp(ox, nx0) * ny0 + mg_clamp(oy, ny0);
+        const T v = mg_jac0(A0, nc0, c, b0);
+        xf[e] = v;
+        if (ox >= 4 * X0 && ox < 4 * (X0 + TX) && ox < nx0 && oy >= 4 * Y0 &&
+            oy < 4 * (Y0 + TY) && oy < ny0)
+            x0[c] = v;
+    }
+    __syncthreads();
+    // level-0 residuals of the children of the level-1 region (0 outside)
+    for (int e = threadIdx.x; e < RW * RH; e += BLOCK) {
+        const int i = e / RH, j = e - i * RH;
+        const int gx = rx0 + i, gy = ry0 + j;
+        T r = (T)0;
+        if (gx >= 0 && gx < nx0 && gy >= 0 && gy < ny0) {
+            const int c = gx * ny0 + gy;
+            T acc = (T)0;
+#pragma unroll
+            for (int d = 0; d < 9; ++d) {
+                const int ex = mg_clamp(gx + d / 3 - 1, nx0), ey = mg_clamp(gy + d % 3 - 1, ny0);
+                acc += A0[(size_t)d * nc0 + c] * xf[(ex - fx0) * FH + (ey - fy0)];
+            }
+            r = b0[c] - acc;
+        }
+        rf[e] = r;
+    }
+    __syncthreads();
+    // level-1 right-hand sides: the children's quad sum (r0 + r1) + (r2 + r3)
+    for (int e = threadIdx.x; e < MW * MH; e += BLOCK) {
+        const int p = e / MH, q = e - p * MH;
+        const int li = 2 * p, lj = 2 * q;
+        const T b = (rf[li * RH + lj] + rf[li * RH + lj + 1]) +
+                    (rf[(li + 1) * RH + lj] + rf[(li + 1) * RH + lj + 1]);
+        bm[e] = b;
+        const int gx = mx0 + p, gy = my0 + q;
+        if (gx >= 2 * X0 && gx < 2 * (X0 + TX) && gx < nx1 && gy >= 2 * Y0 &&
+            gy < 2 * (Y0 + TY) && gy < ny1)
+            b1[gx * ny1 + gy] = b;
+    }
+    __syncthreads();
+    // level 1 from zero, at clamped cells
+    for (int e = threadIdx.x; e < MW * MH; e += BLOCK) {
+        const int p = e / MH, q = e - p * MH;
+        const int ox = mx0 + p, oy = my0 + q;
+        const int gx = mg_clamp(ox, nx1), gy = mg_clamp(oy, ny1);
+        const T dg = A1[(size_t)4 * nc1 + gx * ny1 + gy];
+        const T v = dg != (T)0 ? omega * bm[(gx - mx0) * MH + (gy - my0)] / dg : (T)0;
+        xm[e] = v;
+        if (ox >= 2 * X0 && ox < 2 * (X0 + TX) && ox < nx1 && oy >= 2 * Y0 &&
+            oy < 2 * (Y0 + TY) && oy < ny1)
+            x1[gx * ny1 + gy] = v;
+    }
+    __syncthreads();
+    // level-1 residuals of the tile's level-1 cells (0 outside)
+    for (int e = threadIdx.x; e < 4 * TX * TY; e += BLOCK) {
+        const int p = e / (2 * TY), q = e - p * (2 * TY);
+        const int gx = 2 * X0 + p, gy = 2 * Y0 + q;
+        T r = (T)0;
+        if (gx < nx1 && gy < ny1) {
+            const int c = gx * ny1 + gy;
+            T acc = (T)0;
+#pragma unroll
+            for (int d = 0; d < 9; ++d) {
+                const int ex = mg_clamp(gx + d / 3 - 1, nx1), ey = mg_clamp(gy + d % 3 - 1, ny1);
+                acc += A1[(size_t)d * nc1 + c] * xm[(ex - mx0) * MH + (ey - my0)];
+            }
+            r = bm[(gx - mx0) * MH + (gy - my0)] - acc;
+        }
+        rm[e] = r;
+    }
+    __syncthreads();
+    // level-2 right-hand sides
+    if (threadIdx.x < TX * TY) {
+        const int P = threadIdx.x / TY, Q = threadIdx.x - P * TY;
+        const int X = X0 + P, Y = Y0 + Q;
+        if (X < nx2 && Y < ny2) {
+            const int li = 2 * P, lj = 2 * Q;
+            b2[X * ny2 + Y] = (rm[li * 2 * TY + lj] + rm[li * 2 * TY + lj + 1]) +
+                              (rm[(li + 1) * 2 * TY + lj] + rm[(li + 1) * 2 * TY + lj + 1]);
+        }
+    }
+}
+
+// k_mg_grid_up for levels 1 and 0 of the pair: level 1 corrected from
+// e2 (level 2's result) and swept into out1, then level 0 corrected from
+// out1 and swept into out0.  The tile owns the level-1 cells
+// [2 X0, 2 (X0+TX)) x [2 Y0, 2 (Y0+TY)) and their level-0 children.
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_mg_up2(int nx0, int ny0, const T* __restrict__ A0,
+                                                  const T* __restrict__ b0,
+                                                  const T* __restrict__ x0, int nx1, int ny1,
+                                                  const T* __restrict__ A1,
+                                                  const T* __restrict__ b1,
+                                                  const T* __restrict__ x1, int nx2, int ny2,
+                                                  const T* __restrict__ e2, float over,
+                                                  T* __restrict__ out1, T* __restrict__ out0,
+                                                  const SolverScalars* sc) {
+    if (sc->done) return;
+    constexpr int TX = MG_TX, TY = MG_TY;
+    constexpr int FW = 4 * TX + 2, FH = 4 * TY + 2;  // level 0 under the tile, halo 1
+    constexpr int MW = 2 * TX + 2, MH = 2 * TY + 2;  // their level-1 parents
+    constexpr int ZW = MW + 2, ZH = MH + 2;          // ... with a halo of 1
+    __shared__ T z1[ZW * ZH];
+    __shared__ T o1[MW * MH];
+    __shared__ T z0[FW * FH];
+    const int nty = (ny2 + TY - 1) / TY;
+    const int X0 = (blockIdx.x / nty) * TX, Y0 = (blockIdx.x % nty) * TY;
+    const int fx0 = 4 * X0 - 1, fy0 = 4 * Y0 - 1;  // origin of the level-0 region
+    const int mx0 = 2 * X0 - 1, my0 = 2 * Y0 - 1;  // origin of the level-1 region
+    const int zx0 = mx0 - 1, zy0 = my0 - 1;        // ... with its halo
+    const int nc0 = nx0 * ny0, nc1 = nx1 * ny1;
+    const T f = (T)over;
+    const T omega = mg_omega<T>();
+    // level 1 corrected, z = x + over e(parent), at clamped cells
+    for (int e = threadIdx.x; e < ZW * ZH; e += BLOCK) {
+        const int i = e / ZH, j = e - i * ZH;
+        const int ex = mg_clamp(zx0 + i, nx1), ey = mg_clamp(zy0 + j, ny1);
+        z1[e] = x1[ex * ny1 + ey] + f * e2[(ex >> 1) * ny2 + (ey >> 1)];
+    }
+    __syncthreads();
+    // level-1 sweep over its region; the tile's own cells are written
+    for (int e = threadIdx.x; e < MW * MH; e += BLOCK) {
+        const int p = e / MH, q = e - p * MH;
+        const int gx = mx0 + p, gy = my0 + q;
+        T v = (T)0;
+        if (gx >= 0 && gx < nx1 && gy >= 0 && gy < ny1) {
+            const int c = gx * ny1 + gy;
+            T acc = (T)0;
+#pragma unroll
+            for (int d = 0; d < 9; ++d) {
+                const int ex = mg_clamp(gx + d / 3 - 1, nx1), ey = mg_clamp(gy + d % 3 - 1, ny1);
+                acc += A1[(size_t)d * nc1 + c] * z1[(ex - zx0) * ZH + (ey - zy0)];
+            }
+            const T dg = A1[(size_t)4 * nc1 + c];
+            const T zc = z1[(gx - zx0) * ZH + (gy - zy0)];
+            v = dg != (T)0 ? zc + omega * (b1[c] - acc) / dg : (T)0;
+            if (gx >= 2 * X0 && gx < 2 * (X0 + TX) && gy >= 2 * Y0 && gy < 2 * (Y0 + TY))
+                out1[c] = v;
+        }
+        o1[e] = v;
+    }
+    __syncthreads();
+    // level 0 corrected from the new level-1 values, at clamped cells
+    for (int e = threadIdx.x; e < FW * FH; e += BLOCK) {
+        const int i = e / FH, j = e - i * FH;
+        const int ex = mg_clamp(fx0 + i, nx0), ey = mg_clamp(fy0 + j, ny0);
+        z0[e] = x0[ex * ny0 + ey] + f * o1[((ex >> 1) - mx0) * MH + ((ey >> 1) - my0)];
+    }
+    __syncthreads();
+    // level-0 sweep of the tile's cells
+    for (int e = threadIdx.x; e < 16 * TX * TY; e += BLOCK) {
+        const int p = e / (4 * TY), q = e - p * (4 * TY);
+        const int gx = 4 * X0 + p, gy = 4 * Y0 + q;
+        if (gx >= nx0 || gy >= ny0) continue;
+        const int c = gx * ny0 + gy;
+        T acc = (T)0;
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            const int ex = mg_clamp(gx + d / 3 - 1, nx0), ey = mg_clamp(gy + d % 3 - 1, ny0);
+            acc += A0[(size_t)d * nc0 + c] * z0[(ex - fx0) * FH + (ey - fy0)];
+        }
+        const T dg = A0[(size_t)4 * nc0 + c];
+        const T zc = z0[(gx - fx0) * FH + (gy - fy0)];
+        out0[c] = dg != (T)0 ? zc + omega * (b0[c] - acc) / dg : (T)0;
+    }
+}
+
 // coarse correction of level 0: the particle iterate was 0, so it becomes
 // z = over e1(cell)
 template <class T>
@@ -979,18 +1188,40 @@ static void mg_vcycle(isph_ctx* c, const MGSystem& sys, const TI* nbr, const dou
     k_mg_restrict1<T><<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
         g, c->bincnt, as<T>(m->z0), as<T>(m->lv[0].b), c->sc);
     const int lb = m->lb;
-    for (int l = 0; l < lb; ++l) {
+    // the grid levels down, two per launch (k_mg_down2), the last one alone
+    // when their number is odd
+    int l = 0;
+    for (; l + 1 < lb; l += 2) {
+        MGLevel &a = m->lv[l], &b = m->lv[l + 1], &d = m->lv[l + 2];
+        const int tiles = ((d.nx + MG_TX - 1) / MG_TX) * ((d.ny + MG_TY - 1) / MG_TY);
+        k_mg_down2<T><<<tiles, BLOCK, 0, c->stream>>>(
+            a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.nx, b.ny, as<T>(b.A), as<T>(b.b),
+            as<T>(b.x), d.nx, d.ny, as<T>(d.b), c->sc);
+    }
+    if (l < lb) {
         MGLevel &a = m->lv[l], &b = m->lv[l + 1];
         k_mg_down<T><<<nblk(4 * (int64_t)b.nx * b.ny), BLOCK, 0, c->stream>>>(
             a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.nx, b.ny, as<T>(b.b), c->sc);
     }
     k_mg_block<T><<<1, MG_CMAX, 0, c->stream>>>(m->blk, as<T>(m->lv[lb].b), as<T>(m->lv[lb].x),
                                                  as<T>(m->ainv), m->aok, c->sc);
-    for (int l = lb - 1; l >= 0; --l) {
+    // up: the odd level alone first, then pairs (k_mg_up2)
+    l = lb - 1;
+    if (lb & 1) {
         MGLevel &a = m->lv[l], &b = m->lv[l + 1];
         k_mg_grid_up<T><<<nblk((int64_t)a.nx * a.ny), BLOCK, 0, c->stream>>>(
             a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.ny, as<T>(b.x), MG_OVER, as<T>(a.x2),
             c->sc);
+        std::swap(a.x, a.x2);
+        --l;
+    }
+    for (; l >= 1; l -= 2) {
+        MGLevel &a = m->lv[l - 1], &b = m->lv[l], &d = m->lv[l + 1];
+        const int tiles = ((d.nx + MG_TX - 1) / MG_TX) * ((d.ny + MG_TY - 1) / MG_TY);
+        k_mg_up2<T><<<tiles, BLOCK, 0, c->stream>>>(
+            a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.nx, b.ny, as<T>(b.A), as<T>(b.b),
+            as<T>(b.x), d.nx, d.ny, as<T>(d.x), MG_OVER, as<T>(b.x2), as<T>(a.x2), c->sc);
+        std::swap(b.x, b.x2);
         std::swap(a.x, a.x2);
     }
     k_mg_prolong1<T><<<nblk(n), BLOCK, 0, c->stream>>>(n, m->cell, as<T>(m->lv[0].x), MG_OVER,
