@@ -767,7 +767,10 @@ __global__ void __launch_bounds__(BLOCK) k_mg_down2(int nx0, int ny0, const T* _
 // e2 (level 2's result) and swept into out1, then level 0 corrected from
 // out1 and swept into out0.  The tile owns the level-1 cells
 // [2 X0, 2 (X0+TX)) x [2 Y0, 2 (Y0+TY)) and their level-0 children.
-template <class T>
+// PROLONG: level 0 is the cell grid, and the tile also writes the particle
+// level's corrected iterate z = over out0(cell) for the particles of its
+// cells, as k_mg_prolong1 does.
+template <class T, bool PROLONG>
 __global__ void __launch_bounds__(BLOCK) k_mg_up2(int nx0, int ny0, const T* __restrict__ A0,
                                                   const T* __restrict__ b0,
                                                   const T* __restrict__ x0, int nx1, int ny1,
@@ -776,7 +779,8 @@ __global__ void __launch_bounds__(BLOCK) k_mg_up2(int nx0, int ny0, const T* __r
                                                   const T* __restrict__ x1, int nx2, int ny2,
                                                   const T* __restrict__ e2, float over,
                                                   T* __restrict__ out1, T* __restrict__ out0,
-                                                  const SolverScalars* sc) {
+                                                  GridInfo g, const int* __restrict__ bstart,
+                                                  T* __restrict__ zp, const SolverScalars* sc) {
     if (sc->done) return;
     constexpr int TX = MG_TX, TY = MG_TY;
     constexpr int FW = 4 * TX + 2, FH = 4 * TY + 2;  // level 0 under the tile, halo 1
@@ -843,7 +847,14 @@ __global__ void __launch_bounds__(BLOCK) k_mg_up2(int nx0, int ny0, const T* __r
         }
         const T dg = A0[(size_t)4 * nc0 + c];
         const T zc = z0[(gx - fx0) * FH + (gy - fy0)];
-        out0[c] = dg != (T)0 ? zc + omega * (b0[c] - acc) / dg : (T)0;
+        const T v = dg != (T)0 ? zc + omega * (b0[c] - acc) / dg : (T)0;
+        out0[c] = v;
+        if constexpr (PROLONG) {
+            int lo, hi;
+            column_bins(g, gx, gy, gy, lo, hi);
+            const T zv = f * v;
+            for (int p = bstart[lo], pe = bstart[hi]; p < pe; ++p) zp[p] = zv;
+        }
     }
 }
 
@@ -1215,17 +1226,30 @@ static void mg_vcycle(isph_ctx* c, const MGSystem& sys, const TI* nbr, const dou
         std::swap(a.x, a.x2);
         --l;
     }
+    // the last pair (ending on the cell grid) also writes the particle
+    // iterate, in place of k_mg_prolong1
+    bool prolonged = false;
     for (; l >= 1; l -= 2) {
         MGLevel &a = m->lv[l - 1], &b = m->lv[l], &d = m->lv[l + 1];
         const int tiles = ((d.nx + MG_TX - 1) / MG_TX) * ((d.ny + MG_TY - 1) / MG_TY);
-        k_mg_up2<T><<<tiles, BLOCK, 0, c->stream>>>(
-            a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.nx, b.ny, as<T>(b.A), as<T>(b.b),
-            as<T>(b.x), d.nx, d.ny, as<T>(d.x), MG_OVER, as<T>(b.x2), as<T>(a.x2), c->sc);
+        if (l == 1) {
+            k_mg_up2<T, true><<<tiles, BLOCK, 0, c->stream>>>(
+                a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.nx, b.ny, as<T>(b.A),
+                as<T>(b.b), as<T>(b.x), d.nx, d.ny, as<T>(d.x), MG_OVER, as<T>(b.x2),
+                as<T>(a.x2), g, c->bincnt, as<T>(m->z0), c->sc);
+            prolonged = true;
+        } else {
+            k_mg_up2<T, false><<<tiles, BLOCK, 0, c->stream>>>(
+                a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.nx, b.ny, as<T>(b.A),
+                as<T>(b.b), as<T>(b.x), d.nx, d.ny, as<T>(d.x), MG_OVER, as<T>(b.x2),
+                as<T>(a.x2), g, c->bincnt, as<T>(m->z0), c->sc);
+        }
         std::swap(b.x, b.x2);
         std::swap(a.x, a.x2);
     }
-    k_mg_prolong1<T><<<nblk(n), BLOCK, 0, c->stream>>>(n, m->cell, as<T>(m->lv[0].x), MG_OVER,
-                                                       as<T>(m->z0), c->sc);
+    if (!prolonged)
+        k_mg_prolong1<T><<<nblk(n), BLOCK, 0, c->stream>>>(n, m->cell, as<T>(m->lv[0].x), MG_OVER,
+                                                           as<T>(m->z0), c->sc);
     k_mg_fine_up<T, TI><<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, val, bout,
                                                            as<T>(m->z0), out, c->sc);
 }
