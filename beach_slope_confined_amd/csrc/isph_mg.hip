@@ -14,7 +14,9 @@
 //   block     the levels from the first with at most MG_CMAX cells down to
 //            a single cell: V-cycles inside one block (k_mg_block).
 // Smoothing: the particle level takes one damped-Jacobi sweep after the
-// coarse correction and none before it (x = 0, the residual is b); the grid
+// coarse correction and none before it (x = 0, the residual is b), so its
+// iterate is constant on cells and the sweep reads per-row sums by cell
+// offset instead of the neighbour lists (k_mg_fine_up); the grid
 // levels one sweep before (from x = 0) and one after; the block levels
 // MG_BLOCK_NU sweeps each way, with max(2, ceil(K / 3)) V-cycles for K block
 // levels.  Every coarse correction is scaled by MG_OVER.  The V-cycle is a fixed linear
@@ -37,6 +39,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "isph_host.h"
@@ -86,7 +89,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_rows9(int n, int ny, size_t ld,
                                                     const double* __restrict__ val,
                                                     const int* __restrict__ cell,
                                                     double* __restrict__ part9,
-                                                    uint32_t* err) {
+                                                    float* __restrict__ p9f, uint32_t* err) {
     const int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const int ci = cell[i];
@@ -126,6 +129,9 @@ __global__ void __launch_bounds__(BLOCK) k_mg_rows9(int n, int ny, size_t ld,
     if (bad) atomicOr(err, ERR_DOMAIN);
 #pragma unroll
     for (int d = 0; d < 9; ++d) part9[(size_t)d * ld + i] = acc[d];
+    if (p9f)
+#pragma unroll
+        for (int d = 0; d < 9; ++d) p9f[(size_t)d * ld + i] = (float)acc[d];
 }
 
 template <class T>
@@ -767,10 +773,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_down2(int nx0, int ny0, const T* _
 // e2 (level 2's result) and swept into out1, then level 0 corrected from
 // out1 and swept into out0.  The tile owns the level-1 cells
 // [2 X0, 2 (X0+TX)) x [2 Y0, 2 (Y0+TY)) and their level-0 children.
-// PROLONG: level 0 is the cell grid, and the tile also writes the particle
-// level's corrected iterate z = over out0(cell) for the particles of its
-// cells, as k_mg_prolong1 does.
-template <class T, bool PROLONG>
+template <class T>
 __global__ void __launch_bounds__(BLOCK) k_mg_up2(int nx0, int ny0, const T* __restrict__ A0,
                                                   const T* __restrict__ b0,
                                                   const T* __restrict__ x0, int nx1, int ny1,
@@ -779,8 +782,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_up2(int nx0, int ny0, const T* __r
                                                   const T* __restrict__ x1, int nx2, int ny2,
                                                   const T* __restrict__ e2, float over,
                                                   T* __restrict__ out1, T* __restrict__ out0,
-                                                  GridInfo g, const int* __restrict__ bstart,
-                                                  T* __restrict__ zp, const SolverScalars* sc) {
+                                                  const SolverScalars* sc) {
     if (sc->done) return;
     constexpr int TX = MG_TX, TY = MG_TY;
     constexpr int FW = 4 * TX + 2, FH = 4 * TY + 2;  // level 0 under the tile, halo 1
@@ -847,62 +849,43 @@ __global__ void __launch_bounds__(BLOCK) k_mg_up2(int nx0, int ny0, const T* __r
         }
         const T dg = A0[(size_t)4 * nc0 + c];
         const T zc = z0[(gx - fx0) * FH + (gy - fy0)];
-        const T v = dg != (T)0 ? zc + omega * (b0[c] - acc) / dg : (T)0;
-        out0[c] = v;
-        if constexpr (PROLONG) {
-            int lo, hi;
-            column_bins(g, gx, gy, gy, lo, hi);
-            const T zv = f * v;
-            for (int p = bstart[lo], pe = bstart[hi]; p < pe; ++p) zp[p] = zv;
-        }
+        out0[c] = dg != (T)0 ? zc + omega * (b0[c] - acc) / dg : (T)0;
     }
 }
 
-// coarse correction of level 0: the particle iterate was 0, so it becomes
-// z = over e1(cell)
+// Level 0 up.  The particle iterate before the sweep is the coarse
+// correction alone (the iterate was 0, with no sweep on the way down), so
+// z_j = over e1(cell of j) is constant on every cell and
+//   (A z)_i = sum_d p9[d][i] z(cell_i + d),
+// p9 being row i's entries summed by the neighbour's cell offset d (the
+// 9 planes of k_mg_rows9, unit diagonal in the centre plane).  The sweep
+// out = z + omega (b - A z) reads 9 values a row in place of the row's
+// neighbour list and gathers, and z is never stored.  A particle's cell is
+// interior (binning clamps to cells 1 .. n-2 in both directions), so the
+// 9 cells around it are in the grid.
 template <class T>
-__global__ void k_mg_prolong1(int n, const int* __restrict__ cell, const T* __restrict__ e1,
-                              float over, T* __restrict__ z, const SolverScalars* sc) {
-    if (sc->done) return;
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) z[i] = (T)over * e1[cell[i]];
-}
-
-// level 0 up: one damped-Jacobi sweep from z, out = z + omega (b - A z)
-template <class T, class TI>
-__global__ void __launch_bounds__(BLOCK) k_mg_fine_up(int n, size_t ld, const TI* __restrict__ nbr,
-                                                      const int* __restrict__ cnt,
-                                                      const T* __restrict__ val,
+__global__ void __launch_bounds__(BLOCK) k_mg_fine_up(int n, size_t ld, int ny,
+                                                      const int* __restrict__ cell,
+                                                      const T* __restrict__ p9,
                                                       const double* __restrict__ b,
-                                                      const T* __restrict__ z,
+                                                      const T* __restrict__ e1, float over,
                                                       double* __restrict__ out,
                                                       const SolverScalars* sc) {
     if (sc->done) return;
     int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
-    const T zi = z[i];
-    T acc = zi;
-    const int c = cnt[i];
-    int k = 0;
-    // four entries' offsets and values are loaded before their gathers, so
-    // that the gathers of a row are in flight together; the sum keeps the
-    // entry order
-    for (; k + 4 <= c; k += 4) {
-        const size_t o = (size_t)k * ld + i;
-        const int j0 = mg_col(nbr[o], i), j1 = mg_col(nbr[o + ld], i);
-        const int j2 = mg_col(nbr[o + 2 * ld], i), j3 = mg_col(nbr[o + 3 * ld], i);
-        const T v0 = val[o], v1 = val[o + ld], v2 = val[o + 2 * ld], v3 = val[o + 3 * ld];
-        const T z0 = z[j0], z1 = z[j1], z2 = z[j2], z3 = z[j3];
-        acc += v0 * z0;
-        acc += v1 * z1;
-        acc += v2 * z2;
-        acc += v3 * z3;
+    const int ci = cell[i];
+    const T f = (T)over;
+    T pv[9], zv[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+        pv[d] = p9[(size_t)d * ld + i];
+        zv[d] = f * e1[ci + (d / 3 - 1) * ny + (d % 3 - 1)];
     }
-    for (; k < c; ++k) {
-        const size_t o = (size_t)k * ld + i;
-        acc += val[o] * z[mg_col(nbr[o], i)];
-    }
-    out[i] = (double)(zi + mg_omega<T>() * ((T)b[i] - acc));
+    T acc = (T)0;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) acc += pv[d] * zv[d];
+    out[i] = (double)(zv[4] + mg_omega<T>() * ((T)b[i] - acc));
 }
 
 // ---------------------------------------------------------------------------
@@ -1053,15 +1036,24 @@ struct MGData {
     int lb = 0;    // first level solved inside one block (k_mg_block)
     MGBlock blk;
     MGLevel lv[MG_MAXLEV];
-    double* z0 = nullptr;  // particle residual r0 on the way down, iterate z on the way up
+    double* z0 = nullptr;  // particle residual r0, restricted to the cells
     int* cell = nullptr;   // level-1 cell of every particle
     double* part9 = nullptr;  // per row, its entries summed by neighbour cell offset (9 x cap)
+    float* p9f = nullptr;     // the same sums rounded to FP32, for the FP32 V-cycle
     double* ainv = nullptr;   // inverse of the exact block level (MG_DMAX^2 values)
     int* aok = nullptr;       // 1 when ainv holds it
 };
 
 template <class T>
 static inline T* as(double* p) { return reinterpret_cast<T*>(p); }
+
+// the row sums by cell offset in the V-cycle's value type
+template <class T>
+static inline const T* mg_p9(const MGData* m);
+template <>
+inline const float* mg_p9<float>(const MGData* m) { return m->p9f; }
+template <>
+inline const double* mg_p9<double>(const MGData* m) { return m->part9; }
 
 // the matrix values the V-cycle and the outer products read
 template <class T>
@@ -1079,7 +1071,7 @@ void mg_free(isph_ctx* c) {
         for (auto q : p)
             if (q) (void)hipFree(q);
     }
-    void* p[] = {m->z0, m->cell, m->part9, m->ainv, m->aok};
+    void* p[] = {m->z0, m->cell, m->part9, m->p9f, m->ainv, m->aok};
     for (auto q : p)
         if (q) (void)hipFree(q);
     delete m;
@@ -1113,6 +1105,7 @@ static int mg_alloc(isph_ctx* c) {
     rc |= dalloc(c, &m->z0, N);
     rc |= dalloc(c, &m->cell, N);
     rc |= dalloc(c, &m->part9, 9 * N);
+    rc |= dalloc(c, &m->p9f, 9 * N);
     rc |= dalloc(c, &m->ainv, (size_t)MG_DMAX * MG_DMAX);
     rc |= dalloc(c, &m->aok, 1);
     if (rc) {
@@ -1165,7 +1158,10 @@ static int mg_setup(isph_ctx* c, const MGSystem& sys) {
     k_mg_cells<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, g, c->key_s, m->cell);
     if (c->n)
         k_mg_rows9<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, g.ny, (size_t)c->cap, c->nbr, c->cnt,
-                                                        sys.val64, m->cell, m->part9, c->err);
+                                                        sys.val64, m->cell, m->part9,
+                                                        std::is_same<T, float>::value ? m->p9f
+                                                                                      : nullptr,
+                                                        c->err);
     k_mg_galerkin1<T><<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
         g, c->bincnt, (size_t)c->cap, m->part9, as<T>(m->lv[0].A));
     for (int l = 0; l + 1 < m->nlev; ++l) {
@@ -1191,9 +1187,6 @@ static void mg_vcycle(isph_ctx* c, const MGSystem& sys, const TI* nbr, const dou
     const GridInfo& g = c->grid;
     const int n = c->n;
     const size_t ld = (size_t)c->cap;
-    const T* val = mg_vals<T>(sys);
-    // r0 shares z0's storage: the restriction has read it before the
-    // prolongation writes z0
     k_mg_bpoint<T, MODE, FIRST><<<nblk(n), BLOCK, 0, c->stream>>>(n, pairs, c->sc, bout,
                                                                    as<T>(m->z0));
     k_mg_restrict1<T><<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
@@ -1226,32 +1219,17 @@ static void mg_vcycle(isph_ctx* c, const MGSystem& sys, const TI* nbr, const dou
         std::swap(a.x, a.x2);
         --l;
     }
-    // the last pair (ending on the cell grid) also writes the particle
-    // iterate, in place of k_mg_prolong1
-    bool prolonged = false;
     for (; l >= 1; l -= 2) {
         MGLevel &a = m->lv[l - 1], &b = m->lv[l], &d = m->lv[l + 1];
         const int tiles = ((d.nx + MG_TX - 1) / MG_TX) * ((d.ny + MG_TY - 1) / MG_TY);
-        if (l == 1) {
-            k_mg_up2<T, true><<<tiles, BLOCK, 0, c->stream>>>(
-                a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.nx, b.ny, as<T>(b.A),
-                as<T>(b.b), as<T>(b.x), d.nx, d.ny, as<T>(d.x), MG_OVER, as<T>(b.x2),
-                as<T>(a.x2), g, c->bincnt, as<T>(m->z0), c->sc);
-            prolonged = true;
-        } else {
-            k_mg_up2<T, false><<<tiles, BLOCK, 0, c->stream>>>(
-                a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.nx, b.ny, as<T>(b.A),
-                as<T>(b.b), as<T>(b.x), d.nx, d.ny, as<T>(d.x), MG_OVER, as<T>(b.x2),
-                as<T>(a.x2), g, c->bincnt, as<T>(m->z0), c->sc);
-        }
+        k_mg_up2<T><<<tiles, BLOCK, 0, c->stream>>>(
+            a.nx, a.ny, as<T>(a.A), as<T>(a.b), as<T>(a.x), b.nx, b.ny, as<T>(b.A), as<T>(b.b),
+            as<T>(b.x), d.nx, d.ny, as<T>(d.x), MG_OVER, as<T>(b.x2), as<T>(a.x2), c->sc);
         std::swap(b.x, b.x2);
         std::swap(a.x, a.x2);
     }
-    if (!prolonged)
-        k_mg_prolong1<T><<<nblk(n), BLOCK, 0, c->stream>>>(n, m->cell, as<T>(m->lv[0].x), MG_OVER,
-                                                           as<T>(m->z0), c->sc);
-    k_mg_fine_up<T, TI><<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, nbr, c->cnt, val, bout,
-                                                           as<T>(m->z0), out, c->sc);
+    k_mg_fine_up<T><<<nblk(n), BLOCK, 0, c->stream>>>(n, ld, g.ny, m->cell, mg_p9<T>(m), bout,
+                                                       as<T>(m->lv[0].x), MG_OVER, out, c->sc);
 }
 
 template <class T, class TI>
