@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/isph_capi.h"
@@ -421,6 +422,7 @@ ResidArgs resid_args(isph_ctx* c, double* x, double* r, const double* val, const
     a.n = c->n;
     a.ld = (size_t)c->cap;
     a.nbr = c->nbr;
+    a.nbr16 = nullptr;
     a.cnt = c->cnt;
     a.dpos = c->dpos;
     a.val = val ? val : c->val64;
@@ -504,6 +506,7 @@ int solve_fp64_t(isph_ctx* c, const TI* nbr, const double* val, const double* rh
     auto ka = krylov_args<double, double, TI>(c, nbr, val, vv, c->rq64, c->rv64);
     const SolveCtl ctl = solve_ctl(c, 0);
     ResidArgs ra = resid_args(c, x, c->v64[9], val, rhs);
+    if constexpr (std::is_same<TI, short>::value) ra.nbr16 = nbr;
     k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 1, c->rq64, c->v64[0]);
     k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, ctl);
     HIPCHK(c, hipGetLastError());
@@ -560,6 +563,7 @@ int solve_mixed_t(isph_ctx* c, const TI* nbr, TX* const* v, typename Pair2<TX>::
     SolveCtl ctl = solve_ctl(c, 1);
     SolveCtl ctl_flush = solve_ctl(c, 2);
     ResidArgs ra = resid_args(c, x, r64);
+    if constexpr (std::is_same<TI, short>::value) ra.nbr16 = nbr;
     k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 0, nullptr, nullptr);
     k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, solve_ctl(c, 0));
     k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_RESID, grid, c->part, c->sc, ctl);

@@ -263,6 +263,7 @@ struct ResidArgs {
     int n;
     size_t ld;
     const int* nbr;
+    const short* nbr16;  // offsets j - i of the same lists when every one fits, else null
     const int* cnt;
     const int* dpos;
     const double* val;

@@ -1042,6 +1042,13 @@ struct MGData {
     float* p9f = nullptr;     // the same sums rounded to FP32, for the FP32 V-cycle
     double* ainv = nullptr;   // inverse of the exact block level (MG_DMAX^2 values)
     int* aok = nullptr;       // 1 when ainv holds it
+    // The block levels' operators and the inverse are read only by k_mg_block,
+    // so they are formed on a side stream while the main stream computes the
+    // first residual; the main stream waits for `formed` before the first
+    // V-cycle (mg_join).
+    hipStream_t side = nullptr;
+    hipEvent_t ready = nullptr, formed = nullptr;
+    bool pending = false;  // side-stream work not yet joined
 };
 
 template <class T>
@@ -1066,6 +1073,7 @@ inline const double* mg_vals<double>(const MGSystem& sys) { return sys.val64; }
 void mg_free(isph_ctx* c) {
     MGData* m = c->mg;
     if (!m) return;
+    if (m->side) (void)hipStreamSynchronize(m->side);
     for (int l = 0; l < m->nlev; ++l) {
         void* p[] = {m->lv[l].A, m->lv[l].b, m->lv[l].x, m->lv[l].x2};
         for (auto q : p)
@@ -1074,6 +1082,9 @@ void mg_free(isph_ctx* c) {
     void* p[] = {m->z0, m->cell, m->part9, m->p9f, m->ainv, m->aok};
     for (auto q : p)
         if (q) (void)hipFree(q);
+    if (m->ready) (void)hipEventDestroy(m->ready);
+    if (m->formed) (void)hipEventDestroy(m->formed);
+    if (m->side) (void)hipStreamDestroy(m->side);
     delete m;
     c->mg = nullptr;
 }
@@ -1108,6 +1119,10 @@ static int mg_alloc(isph_ctx* c) {
     rc |= dalloc(c, &m->p9f, 9 * N);
     rc |= dalloc(c, &m->ainv, (size_t)MG_DMAX * MG_DMAX);
     rc |= dalloc(c, &m->aok, 1);
+    if (!rc && (hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&m->ready, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&m->formed, hipEventDisableTiming) != hipSuccess))
+        rc = ISPH_E_HIP;
     if (rc) {
         mg_free(c);
         return fail(c, ISPH_E_HIP, "multigrid hierarchy allocation failed");
@@ -1164,17 +1179,37 @@ static int mg_setup(isph_ctx* c, const MGSystem& sys) {
                                                         c->err);
     k_mg_galerkin1<T><<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
         g, c->bincnt, (size_t)c->cap, m->part9, as<T>(m->lv[0].A));
-    for (int l = 0; l + 1 < m->nlev; ++l) {
+    // the grid levels' operators on the main stream, the block levels'
+    // (lb and below) and the inverse on the side stream
+    auto galerkin = [&](int l, hipStream_t st) {
         const MGLevel &a = m->lv[l], &b = m->lv[l + 1];
-        k_mg_galerkin<T><<<nblk((int64_t)b.nx * b.ny), BLOCK, 0, c->stream>>>(
-            a.nx, a.ny, as<T>(a.A), b.nx, b.ny, as<T>(b.A));
-    }
+        k_mg_galerkin<T><<<nblk((int64_t)b.nx * b.ny), BLOCK, 0, st>>>(a.nx, a.ny, as<T>(a.A),
+                                                                      b.nx, b.ny, as<T>(b.A));
+    };
+    int l = 0;
+    for (; l + 1 < m->nlev && l + 1 < m->lb; ++l) galerkin(l, c->stream);
+    HIPCHK(c, hipEventRecord(m->ready, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(m->side, m->ready, 0));
+    for (; l + 1 < m->nlev; ++l) galerkin(l, m->side);
     if (m->blk.D) {
         const MGLevel& L = m->lv[m->lb + m->blk.D];
-        k_mg_coarse_inv<T><<<1, MG_CMAX, 0, c->stream>>>(L.nx, L.ny, as<T>(L.A), as<T>(m->ainv),
-                                                          m->aok);
+        k_mg_coarse_inv<T><<<1, MG_CMAX, 0, m->side>>>(L.nx, L.ny, as<T>(L.A), as<T>(m->ainv),
+                                                        m->aok);
     }
+    HIPCHK(c, hipEventRecord(m->formed, m->side));
+    m->pending = true;
     HIPCHK(c, hipGetLastError());
+    return ISPH_OK;
+}
+
+// the main stream waits for the side stream's operators and inverse: before
+// the first V-cycle, and before the solve returns (the next set-up rewrites
+// what the side stream reads)
+static int mg_join(isph_ctx* c) {
+    MGData* m = c->mg;
+    if (!m || !m->pending) return ISPH_OK;
+    m->pending = false;
+    HIPCHK(c, hipStreamWaitEvent(c->stream, m->formed, 0));
     return ISPH_OK;
 }
 
@@ -1241,6 +1276,7 @@ static int mg_iterate(isph_ctx* c, const MGSystem& sys, const TI* nbr, const Sol
     const size_t ld = (size_t)c->cap;
     double *rhat = c->v64[0], *p = c->v64[1], *s = c->v64[2], *t = c->v64[3], *xs = c->v64[4];
     double *ph = c->v64[5], *sh = c->v64[6];
+    if (int rc = mg_join(c)) return rc;
     for (;;) {
         const int before = c->sc_host->total;
         (void)before;
@@ -1276,7 +1312,7 @@ static int mg_iterate(isph_ctx* c, const MGSystem& sys, const TI* nbr, const Sol
 }
 
 template <class T, class TI>
-static int solve_mg_t(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
+static int solve_mg_body(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
     const int grid = nblk8(c->n);
     double* x = sys.x;
     double* r64 = c->v64[9];
@@ -1285,6 +1321,7 @@ static int solve_mg_t(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
     SolveCtl ctl = solve_ctl(c, 1);
     SolveCtl ctl_flush = solve_ctl(c, 2);
     ResidArgs ra = resid_args(c, x, r64, sys.val64, sys.rhs);
+    if constexpr (std::is_same<TI, short>::value) ra.nbr16 = nbr;
     k_resid<<<grid, BLOCK, 0, c->stream>>>(ra, 0, nullptr, nullptr);
     k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_INIT, grid, c->part, c->sc, solve_ctl(c, 0));
     k_finish<<<1, FIN_BLOCK, 0, c->stream>>>(FIN_RESID, grid, c->part, c->sc, ctl);
@@ -1327,6 +1364,13 @@ static int solve_mg_t(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
         HIPCHK(c, hipGetLastError());
         if ((rc = read_sc(c))) return rc;
     }
+}
+
+template <class T, class TI>
+static int solve_mg_t(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
+    const int rc = solve_mg_body<T, TI>(c, sys, nbr);
+    const int rj = mg_join(c);
+    return rc ? rc : rj;
 }
 
 int solve_mg(isph_ctx* c) {

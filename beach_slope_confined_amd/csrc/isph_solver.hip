@@ -346,8 +346,13 @@ __global__ void __launch_bounds__(BLOCK) k_resid(ResidArgs a, int mode, double2*
     double vb = 0.0, vr = 0.0;
     if (i < a.n) {
         const double* __restrict__ x = a.x;
-        const double s = spmv_row<double, double, int>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld,
-                                                       x[i], [&](int j) { return x[j]; });
+        auto xj = [&](int j) { return x[j]; };
+        // the same sum in the same order with either index form
+        const double s =
+            a.nbr16 ? spmv_row<double, double, short>(i, a.cnt[i], a.dpos[i], a.nbr16, a.val, a.ld,
+                                                      x[i], xj)
+                    : spmv_row<double, double, int>(i, a.cnt[i], a.dpos[i], a.nbr, a.val, a.ld,
+                                                    x[i], xj);
         const double bi = a.b[i];
         const double r = bi - s;
         a.r[i] = r;
