@@ -953,7 +953,10 @@ __global__ void __launch_bounds__(BLOCK) k_mg_t(int n, size_t ld, const TI* __re
                                                 double* __restrict__ t, double* __restrict__ part,
                                                 const SolverScalars* sc) {
     if (sc->done) return;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    // rows in the opposite order to k_mg_v: this pass starts on the matrix
+    // rows that k_mg_v read last, and the next k_mg_v on those this one read
+    // last, while they are still in the Infinity Cache
+    const int lb = xcd_block_rev(blockIdx.x, gridDim.x);
     const int i = lb * BLOCK + threadIdx.x;
     double v0 = 0.0, v1 = 0.0;
     if (i < n) {
@@ -1278,8 +1281,6 @@ static int mg_iterate(isph_ctx* c, const MGSystem& sys, const TI* nbr, const Sol
     double *ph = c->v64[5], *sh = c->v64[6];
     if (int rc = mg_join(c)) return rc;
     for (;;) {
-        const int before = c->sc_host->total;
-        (void)before;
         int batch = 4;
         const SolverScalars& h = *c->sc_host;
         if (h.nb > 0 && h.rr > 0) {
