@@ -62,6 +62,8 @@ int check_params(isph_ctx* c, const isph_params* p) {
         return fail(c, ISPH_E_ARG, "unknown cell_order");
     if (p->precision < ISPH_PREC_FP64 || p->precision > ISPH_PREC_FP64_MG)
         return fail(c, ISPH_E_ARG, "unknown precision");
+    if (!(p->inner_tol >= 0.0 && p->inner_tol < 1.0))
+        return fail(c, ISPH_E_ARG, "inner_tol must lie in [0, 1)");
     if (p->cell_order == ISPH_ORDER_REFERENCE && !(p->cell_size >= 4.0 * p->h))
         return fail(c, ISPH_E_ARG, "the reference order needs cell_size >= 4h (2x2-cell blocks)");
     GridInfo g = make_grid(*p);
@@ -410,7 +412,8 @@ void prof_collect(isph_ctx* c) {
 SolveCtl solve_ctl(const isph_ctx* c, int mixed) {
     SolveCtl t;
     t.tol = c->prm.tol;
-    t.delta = c->prm.inner_tol;
+    // inner_tol 0: the precision's default reliable-update factor
+    t.delta = c->prm.inner_tol > 0.0 ? c->prm.inner_tol : prec_mg(c->prm.precision) ? 1e-3 : 0.1;
     t.maxit = c->prm.maxit;
     t.maxupd = c->prm.max_outer;
     t.mixed = mixed;

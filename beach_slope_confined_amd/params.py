@@ -84,7 +84,7 @@ def make_params(**kw):
         kernel=KERNEL_WENDLAND_C2, drag_law=DRAG_ERGUN, visc_law=VISC_CONSTANT,
         precision=PREC_FP64, rho0=1000.0, rel_den=1.025, g=-9.81, nu=1e-6,
         csh=0.5, maxshift=0.1, lamfs=0.8, maxdivr=2.0, por=0.4, alpha=200.0,
-        beta=1.1, d50=0.01, tol=1e-8, inner_tol=0.1, maxit=50000,
+        beta=1.1, d50=0.01, tol=1e-8, inner_tol=0.0, maxit=50000,
         inner_maxit=0, max_outer=200, max_nbr=64, shifting=1, adaptive_dt=0,
         sink=0, xlcutoff=-1e300, ytcutoff=1e300, sig1=0.2, mu=1e-3, salinity=0,
         cell_order=ORDER_CELL, reserved=0,

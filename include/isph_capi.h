@@ -138,7 +138,12 @@ typedef struct isph_params {
     double d50;        /* grain diameter (porous.f90:111)                      */
     double tol;        /* PPE relative residual tolerance ||b-Ax||/||b||        */
     double inner_tol;  /* mixed precision: reliable-update factor delta (the FP64
-                          residual replaces the FP32 one each time it fell by delta) */
+                          residual replaces the FP32 one each time it fell by
+                          delta); ISPH_PREC_IR: the corrections' reduction.
+                          0 selects the precision's default: 0.1, and 1e-3 for
+                          MIXED_MG and FP64_MG (whose reliable updates cost a
+                          FP64 residual pass against about 8 iterations'
+                          reduction).  Range [0, 1).                         */
     double xlcutoff;   /* outflow sink (integrator.f90:60-61), active if sink  */
     double ytcutoff;
     double sig1;       /* adaptive dt factor (initialize.f90:35)               */

@@ -1260,7 +1260,7 @@ static void spmv_csr_fd(int32_t n, const int32_t* rp, const int32_t* col, const 
  * Krylov vectors of type TX (float: ISPH_PREC_MIXED, double:
  * ISPH_PREC_MIXED_V64), dot products accumulated in FP64, and reliable
  * updates (Sleijpen & van der Vorst 1996; van der Vorst & Ye 2000): when
- * the recurrence residual has dropped by delta = inner_tol since the last
+ * the recurrence residual has dropped by delta = inner_tol (0: 0.1) since the last
  * update (or below tol), the iterate is added to the FP64 solution, the
  * FP64 true residual b - A64 x replaces the recurrence residual and the
  * iteration continues with its search direction kept.  The stop test is the
@@ -1269,6 +1269,8 @@ static void spmv_csr_fd(int32_t n, const int32_t* rp, const int32_t* col, const 
 static int NAME(ora_ctx* c, const float* val32, double* x, int32_t* iters, int32_t* outer,  \
                 double* relres) {                                                           \
     const isph_params* prm = &c->prm;                                                       \
+    /* inner_tol 0: the default factor 0.1 (isph_capi.h) */                                 \
+    const double delta = prm->inner_tol > 0.0 ? prm->inner_tol : 0.1;                       \
     const int32_t n = c->n;                                                                 \
     double* r64 = c->w[6];                                                                  \
     double* ax = c->w[7];                                                                   \
@@ -1334,7 +1336,7 @@ static int NAME(ora_ctx* c, const float* val32, double* x, int32_t* iters, int32
         prevr = d4;                                                                         \
         if (!(fabs(d4) > BRK_COS * sqrt(rrk * rh2))) brk = 1;                               \
         if (!brk && (!isfinite(betas) || !isfinite(rrk))) { rc = ISPH_E_NOCONV; break; }    \
-        if (brk || sqrt(rrk) <= prm->tol * nb || sqrt(rrk) <= prm->inner_tol * rupd) {      \
+        if (brk || sqrt(rrk) <= prm->tol * nb || sqrt(rrk) <= delta * rupd) {               \
             for (int32_t i = 0; i < n; ++i) {                                               \
                 x[i] = x[i] + (double)sl[i];                                                \
                 sl[i] = (TX)0;                                                              \

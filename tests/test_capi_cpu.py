@@ -75,3 +75,14 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(capi, "_LIB", None)
     with pytest.raises(ImportError):
         capi.load_library(str(tmp_path / "absent.so"))
+
+
+@pytest.mark.parametrize("bad", [-0.1, 1.0, float("nan")])
+def test_inner_tol_outside_its_range_is_rejected(bad):
+    """inner_tol lies in [0, 1) (0: the precision's default); the parameter
+    check runs before any device call, so this needs no GPU."""
+    lib = capi.load_library()
+    prm = configs.dam_break(0.01)["params"].copy(inner_tol=bad)
+    ctx = ctypes.c_void_p()
+    rc = lib.isph_create(ctypes.byref(prm), 1000, 0, ctypes.byref(ctx))
+    assert rc == -1 and not ctx.value

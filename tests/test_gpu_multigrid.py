@@ -115,3 +115,22 @@ def test_multigrid_solve_after_a_failed_one_in_another_context(require_gpu):
     print(f"before: {it0} iterations, {rel0:.3e}; after: {it1}, {rel1:.3e}", flush=True)
     assert (it1, rel1) == (it0, rel0)
     assert np.array_equal(x1, x0)
+
+
+def test_multigrid_default_reliable_update_factor(require_gpu):
+    """inner_tol 0 selects the multigrid precisions' default factor 1e-3:
+    the same solve bit for bit as an explicit 1e-3, with fewer reliable
+    updates than the unpreconditioned forms' 0.1."""
+    cfg = CASES["tank_128"]()
+    res = {}
+    for d in (0.0, 1e-3, 0.1):
+        c = dict(cfg)
+        c["params"] = cfg["params"].copy(inner_tol=d)
+        sim = _to_solve(c, PREC_MIXED_MG, 1e-9)
+        it, rel = sim.ppe_solve()
+        res[d] = (it, rel, sim.scalars()["outer"], sim.get_field("pressure"))
+        sim.close()
+    print({d: r[:3] for d, r in res.items()}, flush=True)
+    assert res[0.0][:3] == res[1e-3][:3]
+    assert np.array_equal(res[0.0][3], res[1e-3][3])
+    assert res[1e-3][2] < res[0.1][2]
