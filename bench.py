@@ -419,18 +419,24 @@ def host_cores():
 
 
 def model_mg_step_bytes(n, nnz_off, iters, val_bytes=4):
-    """Algorithmic bytes of one multigrid-preconditioned step: the fixed
-    stages 1284 + 60 K per row (the FP64 step's model) plus, per BiCGSTAB
-    iteration, two V-cycles (the particle-level down and up passes each
-    read the row's offsets and values once, 6 K with FP32 values, plus
-    36 and 24 bytes of own vectors; restriction, prolongation and the cell
-    levels about 26 bytes), the two outer products A p^, A s^ (6 K + 60,
-    6 K + 36) and the update pass (96): 36 K + 364 per row with FP32 values.
-    With FP64 values (val_bytes 8) each of the six matrix passes reads
-    4 K more and the V-cycle's own vectors 8 more."""
+    """Algorithmic bytes of one multigrid-preconditioned step, per row: the
+    fixed stages 1284 + 60 K (the FP64 step's model) plus, per BiCGSTAB
+    iteration, with v = val_bytes (4 for FP32 values, 8 for FP64):
+    - two V-cycles, each: the point right-hand side (a 16-byte pair read,
+      8 + v written), its restriction to the cells (v), the cell levels
+      (about 23 v / 4: nine stencil values and three vector values per cell
+      and level, down and up, on a fifth as many cells as rows) and the
+      particle sweep (nine cell-offset sums, the cell, b and the output:
+      9 v + 20);
+    - the two outer products A p^ and A s^, the only passes over the
+      matrix: K (v + 2) + 60 and K (v + 2) + 36;
+    - the update pass (96).
+    With FP32 values that is 12 K + 414 bytes a row per iteration."""
     k = nnz_off / n
-    extra = (val_bytes - 4) * (6 * k + 2)
-    return n * ((1284 + 60 * k) + iters * (36 * k + 364 + extra))
+    v = val_bytes
+    vcycle = (24 + v) + v + 23 * v / 4 + (9 * v + 20)
+    per_iter = 2 * vcycle + 2 * k * (v + 2) + 60 + 36 + 96
+    return n * ((1284 + 60 * k) + iters * per_iter)
 
 
 def run_mixed_errors():
@@ -556,7 +562,7 @@ def run_large(steps, with_adjoint):
                       "umax_at_end": s["umax"], "t_at_end": s["t"]}
     out["step_model"] = {"bytes_per_step": round(sb), "achieved": round(sb / (sec / steps) / 1e9, 1),
                          "unit": "GB/s", "frac": round(sb / (sec / steps) / 1e9 / HBM_PEAK_GBS, 4),
-                         "model": "rows x (1284 + 60K + iterations x (36K + 364)) bytes, "
+                         "model": "rows x (1284 + 60K + iterations x (12K + 414)) bytes, "
                                   "K = %.3f" % (nnz / n)}
     if with_adjoint:
         w, _hc = big["column"]
@@ -814,7 +820,7 @@ def main():
             line["mixed_mg"]["step_model"] = {
                 "bytes_per_step": round(sb), "achieved": round(ach, 1), "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4),
-                "model": "rows x (1284 + 60K + iterations x (36K + 364)) bytes, K = %.3f"
+                "model": "rows x (1284 + 60K + iterations x (12K + 414)) bytes, K = %.3f"
                          % (r64["nnz_off"] / r64["n"])}
         except Exception as e:  # reported, not hidden
             line["mixed_mg"] = {"error": str(e)}
