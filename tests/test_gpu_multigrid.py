@@ -134,3 +134,30 @@ def test_multigrid_default_reliable_update_factor(require_gpu):
     assert res[0.0][:3] == res[1e-3][:3]
     assert np.array_equal(res[0.0][3], res[1e-3][3])
     assert res[1e-3][2] < res[0.1][2]
+
+
+def test_multigrid_solve_after_one_that_ends_before_a_v_cycle(require_gpu):
+    """The block levels' operators and the coarse inverse are formed on a
+    side stream and joined before the first V-cycle.  A solve with a zero
+    right-hand side ends before any V-cycle; it must still join, so that the
+    next solve in the same context (whose set-up rewrites those operators)
+    converges bit for bit as the same solve in a fresh context."""
+    cfg = CASES["dam_break"]()
+    fresh = _to_solve(cfg, PREC_MIXED_MG, 1e-9)
+    fresh.set_field("pressure", np.zeros(fresh.count()))
+    it0, rel0 = fresh.ppe_solve()
+    x0 = fresh.get_field("pressure")
+    fresh.close()
+    sim = _to_solve(cfg, PREC_MIXED_MG, 1e-9)
+    rhs = sim.get_field("rhs")
+    sim.set_field("rhs", np.zeros_like(rhs))
+    it_z, _ = sim.ppe_solve()
+    assert it_z == 0
+    assert not np.any(sim.get_field("pressure"))
+    sim.set_field("rhs", rhs)
+    it1, rel1 = sim.ppe_solve()
+    x1 = sim.get_field("pressure")
+    sim.close()
+    print(f"fresh: {it0} iterations, {rel0:.3e}; after the zero solve: {it1}, {rel1:.3e}", flush=True)
+    assert (it1, rel1) == (it0, rel0)
+    assert np.array_equal(x1, x0)
