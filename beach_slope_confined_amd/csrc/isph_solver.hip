@@ -165,8 +165,10 @@ __global__ void __launch_bounds__(BLOCK) k_krylov_3(KrylovArgs<TV, TX, TI> a,
 __global__ void __launch_bounds__(FIN_BLOCK) k_finish(int kind, int nbp,
                                                       const double* __restrict__ part,
                                                       SolverScalars* sc, SolveCtl ctl) {
-    if ((kind == FIN_ALPHA || kind == FIN_OMEGA || kind == FIN_BETA) && sc->done) return;
     const bool two = kind != FIN_ALPHA && kind != FIN_START && kind != FIN_REPLACE;
+    // The partials and the solver state are loaded before anything waits on
+    // either, so neither the done test nor the update below adds its load
+    // latency to the chain; the update works on this copy and stores it.
     double p0[8], p1[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -174,6 +176,8 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_finish(int kind, int nbp,
         p0[u] = k < nbp ? part[k] : 0.0;
         p1[u] = (two && k < nbp) ? part[nbp + k] : 0.0;
     }
+    SolverScalars s = *sc;
+    if ((kind == FIN_ALPHA || kind == FIN_OMEGA || kind == FIN_BETA) && s.done) return;
     double s0 = 0.0, s1 = 0.0;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -191,25 +195,25 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_finish(int kind, int nbp,
     if (threadIdx.x != 0) return;
     switch (kind) {
         case FIN_INIT: {
-            sc->bb = t0;
-            sc->nb = sqrt(t0);
-            sc->rr = t1;
-            sc->prevr = t1;
-            sc->iters = 0;
-            sc->total = 0;
-            sc->first = 1;
-            sc->status = 0;
-            sc->nupd = 0;
-            sc->rh2 = t1;
-            sc->brk = 0;
+            s.bb = t0;
+            s.nb = sqrt(t0);
+            s.rr = t1;
+            s.prevr = t1;
+            s.iters = 0;
+            s.total = 0;
+            s.first = 1;
+            s.status = 0;
+            s.nupd = 0;
+            s.rh2 = t1;
+            s.brk = 0;
             if (t0 == 0.0) {
-                sc->done = DONE_ZERO_RHS;
-                sc->relres = 0.0;
-            } else if (sqrt(t1) <= ctl.tol * sc->nb) {
-                sc->done = DONE_CONVERGED;
-                sc->relres = sqrt(t1) / sc->nb;
+                s.done = DONE_ZERO_RHS;
+                s.relres = 0.0;
+            } else if (sqrt(t1) <= ctl.tol * s.nb) {
+                s.done = DONE_CONVERGED;
+                s.relres = sqrt(t1) / s.nb;
             } else {
-                sc->done = 0;
+                s.done = 0;
             }
             break;
         }
@@ -217,124 +221,125 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_finish(int kind, int nbp,
         // degenerates to a minimal-residual step (alpha = 0) or to the half
         // step x + alpha p (omega = 0) and ends in a restart
         case FIN_ALPHA: {
-            double a = sc->prevr / t0;
+            double a = s.prevr / t0;
             if (!isfinite(a)) {
                 a = 0.0;
-                sc->brk = 1;
+                s.brk = 1;
             }
-            sc->alpha = a;
+            s.alpha = a;
             break;
         }
         case FIN_OMEGA: {
             double w = t0 / t1;
             if (!isfinite(w) || w == 0.0) {
                 w = 0.0;
-                sc->brk = 1;
+                s.brk = 1;
             }
-            sc->wi = w;
+            s.wi = w;
             break;
         }
         case FIN_BETA: {
-            const double betas = (t0 / sc->prevr) * (sc->alpha / sc->wi);
-            sc->betas = betas;
-            sc->prevr_prev = sc->prevr;
-            sc->prevr = t0;
-            sc->rr = t1;
-            sc->iters = sc->iters + 1;
-            sc->total = sc->total + 1;
-            sc->first = 0;
+            const double betas = (t0 / s.prevr) * (s.alpha / s.wi);
+            s.betas = betas;
+            s.prevr_prev = s.prevr;
+            s.prevr = t0;
+            s.rr = t1;
+            s.iters = s.iters + 1;
+            s.total = s.total + 1;
+            s.first = 0;
             const double nr = sqrt(t1);
-            if (!(fabs(t0) > BRK_COS * sqrt(t1 * sc->rh2))) sc->brk = 1;
-            if (sc->brk) {
+            if (!(fabs(t0) > BRK_COS * sqrt(t1 * s.rh2))) s.brk = 1;
+            if (s.brk) {
                 // FP64: true residual + restart; mixed: flush + full restart
                 // (the flag stays set for the host until FIN_START)
                 if (!ctl.mixed) {
-                    sc->brk = 0;
-                    sc->done = DONE_RECURRENCE;
+                    s.brk = 0;
+                    s.done = DONE_RECURRENCE;
                 } else {
-                    sc->done = DONE_UPDATE;
+                    s.done = DONE_UPDATE;
                 }
             } else if (!isfinite(betas) || !isfinite(t1)) {
-                sc->done = DONE_FAILED;
-                sc->status = ERR_NOCONV;
-                sc->relres = nr / sc->nb;
-            } else if (!ctl.mixed && nr <= ctl.tol * sc->nb) {
-                sc->done = DONE_RECURRENCE;
-            } else if (ctl.mixed && (nr <= ctl.tol * sc->nb || nr <= ctl.delta * sc->rupd)) {
-                sc->done = DONE_UPDATE;
-            } else if (sc->total >= ctl.maxit) {
-                sc->done = DONE_FAILED;
-                sc->status = ERR_NOCONV;
-                sc->relres = nr / sc->nb;
+                s.done = DONE_FAILED;
+                s.status = ERR_NOCONV;
+                s.relres = nr / s.nb;
+            } else if (!ctl.mixed && nr <= ctl.tol * s.nb) {
+                s.done = DONE_RECURRENCE;
+            } else if (ctl.mixed && (nr <= ctl.tol * s.nb || nr <= ctl.delta * s.rupd)) {
+                s.done = DONE_UPDATE;
+            } else if (s.total >= ctl.maxit) {
+                s.done = DONE_FAILED;
+                s.status = ERR_NOCONV;
+                s.relres = nr / s.nb;
             }
             break;
         }
         case FIN_TRUE: {
             const double nr = sqrt(t1);
-            if (nr <= ctl.tol * sc->nb) {
-                sc->done = DONE_CONVERGED;
-                sc->relres = nr / sc->nb;
+            if (nr <= ctl.tol * s.nb) {
+                s.done = DONE_CONVERGED;
+                s.relres = nr / s.nb;
             } else if (!isfinite(nr)) {
-                sc->done = DONE_FAILED;
-                sc->status = ERR_NOCONV;
-                sc->relres = nr;
+                s.done = DONE_FAILED;
+                s.status = ERR_NOCONV;
+                s.relres = nr;
             } else {
-                sc->rr = t1;
-                sc->prevr = t1;
-                sc->rh2 = t1;
-                sc->first = 1;
-                sc->done = 0;
-                if (sc->total >= ctl.maxit) {
-                    sc->done = DONE_FAILED;
-                    sc->status = ERR_NOCONV;
-                    sc->relres = nr / sc->nb;
+                s.rr = t1;
+                s.prevr = t1;
+                s.rh2 = t1;
+                s.first = 1;
+                s.done = 0;
+                if (s.total >= ctl.maxit) {
+                    s.done = DONE_FAILED;
+                    s.status = ERR_NOCONV;
+                    s.relres = nr / s.nb;
                 }
             }
             break;
         }
         case FIN_RESID: {
             // mixed outer residual; nupd counts the flushes (ctl.mixed == 2 after a flush)
-            sc->bb = t0;
-            sc->nb = sqrt(t0);
-            sc->nr = sqrt(t1);
-            if (ctl.mixed == 2) sc->nupd = sc->nupd + 1;
+            s.bb = t0;
+            s.nb = sqrt(t0);
+            s.nr = sqrt(t1);
+            if (ctl.mixed == 2) s.nupd = s.nupd + 1;
             if (t0 == 0.0) {
-                sc->done = DONE_ZERO_RHS;
-                sc->relres = 0.0;
+                s.done = DONE_ZERO_RHS;
+                s.relres = 0.0;
                 break;
             }
-            sc->relres = sc->nr / sc->nb;
-            if (sc->nr <= ctl.tol * sc->nb) {
-                sc->done = DONE_CONVERGED;
+            s.relres = s.nr / s.nb;
+            if (s.nr <= ctl.tol * s.nb) {
+                s.done = DONE_CONVERGED;
             } else if (ctl.mixed == 2 &&
-                       (sc->nupd >= ctl.maxupd || sc->total >= ctl.maxit || !isfinite(sc->nr))) {
-                sc->done = DONE_FAILED;
-                sc->status = ERR_NOCONV;
+                       (s.nupd >= ctl.maxupd || s.total >= ctl.maxit || !isfinite(s.nr))) {
+                s.done = DONE_FAILED;
+                s.status = ERR_NOCONV;
             } else {
-                sc->done = 0;
+                s.done = 0;
             }
             break;
         }
         case FIN_START:
-            sc->prevr = t0;
-            sc->prevr_prev = t0;
-            sc->rh2 = t0;
-            sc->brk = 0;
-            sc->rupd = sc->nr;
-            sc->first = 1;
-            sc->iters = 0;
-            sc->done = 0;
+            s.prevr = t0;
+            s.prevr_prev = t0;
+            s.rh2 = t0;
+            s.brk = 0;
+            s.rupd = s.nr;
+            s.first = 1;
+            s.iters = 0;
+            s.done = 0;
             break;
         case FIN_REPLACE: {
-            sc->betas = (t0 / sc->prevr_prev) * (sc->alpha / sc->wi);
-            sc->prevr = t0;
-            sc->rupd = sc->nr;
-            sc->done = 0;
+            s.betas = (t0 / s.prevr_prev) * (s.alpha / s.wi);
+            s.prevr = t0;
+            s.rupd = s.nr;
+            s.done = 0;
             break;
         }
         default:
             break;
     }
+    *sc = s;
 }
 
 // r = b - A64 x in FP64; partials (b,b), (r,r).  mode 1 also restarts the
