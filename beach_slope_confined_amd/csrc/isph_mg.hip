@@ -186,11 +186,13 @@ __global__ void k_mg_galerkin(int nx, int ny, const T* __restrict__ A, int nx2, 
 template <class T, int MODE, bool FIRST>
 __global__ void k_mg_bpoint(int n, const double2* __restrict__ pairs, const SolverScalars* sc,
                             double* __restrict__ bout, T* __restrict__ r0) {
-    if (sc->done) return;
+    // the flag, the coefficient and the pair are requested together
+    const int done = sc->done;
+    const double coef = MODE == 0 ? sc->betas : sc->alpha;
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    const double coef = MODE == 0 ? sc->betas : sc->alpha;
     const double2 u = pairs[i];
+    if (done) return;
     const double bi = MODE == 0 ? (FIRST ? u.x : u.x + coef * u.y) : u.x - coef * u.y;
     bout[i] = bi;
     r0[i] = (T)bi;
@@ -871,10 +873,11 @@ __global__ void __launch_bounds__(BLOCK) k_mg_fine_up(int n, size_t ld, int ny,
                                                       const T* __restrict__ e1, float over,
                                                       double* __restrict__ out,
                                                       const SolverScalars* sc) {
-    if (sc->done) return;
+    const int done = sc->done;  // tested once the row's cell is requested
     int i = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     const int ci = cell[i];
+    if (done) return;
     const T f = (T)over;
     T pv[9], zv[9];
 #pragma unroll
@@ -929,12 +932,20 @@ __global__ void __launch_bounds__(BLOCK) k_mg_v(int n, size_t ld, const TI* __re
                                                 const double* __restrict__ rhat,
                                                 double2* __restrict__ rv, double* __restrict__ part,
                                                 const SolverScalars* sc) {
-    if (sc->done) return;
+    // the row's count and row sum are requested before the done test waits
+    const int done = sc->done;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int i = lb * BLOCK + threadIdx.x;
+    int ci = 0;
+    double si = 0.0;
+    if (i < n) {
+        ci = cnt[i];
+        si = rsum[i];
+    }
+    if (done) return;
     double v = 0.0;
     if (i < n) {
-        const double s = mg_spmv<TI, TV>(i, cnt[i], rsum[i], nbr, val, ld, ph);
+        const double s = mg_spmv<TI, TV>(i, ci, si, nbr, val, ld, ph);
         rv[i] = make_double2(rq[i].x, s);
         v = s * rhat[i];
     }
@@ -952,15 +963,22 @@ __global__ void __launch_bounds__(BLOCK) k_mg_t(int n, size_t ld, const TI* __re
                                                 const double* __restrict__ s,
                                                 double* __restrict__ t, double* __restrict__ part,
                                                 const SolverScalars* sc) {
-    if (sc->done) return;
+    const int done = sc->done;  // tested once the row's first loads are requested
     // rows in the opposite order to k_mg_v: this pass starts on the matrix
     // rows that k_mg_v read last, and the next k_mg_v on those this one read
     // last, while they are still in the Infinity Cache
     const int lb = xcd_block_rev(blockIdx.x, gridDim.x);
     const int i = lb * BLOCK + threadIdx.x;
+    int ci = 0;
+    double si = 0.0;
+    if (i < n) {
+        ci = cnt[i];
+        si = rsum[i];
+    }
+    if (done) return;
     double v0 = 0.0, v1 = 0.0;
     if (i < n) {
-        const double ti = mg_spmv<TI, TV>(i, cnt[i], rsum[i], nbr, val, ld, sh);
+        const double ti = mg_spmv<TI, TV>(i, ci, si, nbr, val, ld, sh);
         t[i] = ti;
         v0 = ti * s[i];
         v1 = ti * ti;
@@ -983,13 +1001,21 @@ __global__ void __launch_bounds__(BLOCK) k_mg_x(int n, const double* __restrict_
                                                 const double* __restrict__ rhat,
                                                 double* __restrict__ xs, double2* __restrict__ rq,
                                                 double* __restrict__ part, const SolverScalars* sc) {
-    if (sc->done) return;
+    // the scalars and the row's first operands are requested before the
+    // done test waits
+    const int done = sc->done;
+    const double al = sc->alpha, w = sc->wi;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int i = lb * BLOCK + threadIdx.x;
+    double xi = 0.0, phi = 0.0;
+    if (i < n) {
+        xi = xs[i];
+        phi = ph[i];
+    }
+    if (done) return;
     double v0 = 0.0, v1 = 0.0;
     if (i < n) {
-        const double al = sc->alpha, w = sc->wi;
-        xs[i] = (xs[i] + al * ph[i]) + w * sh[i];
+        xs[i] = (xi + al * phi) + w * sh[i];
         const double r = s[i] - w * t[i];
         const double q = p[i] - w * rv[i].y;
         rq[i] = make_double2(r, q);

@@ -73,18 +73,22 @@ template <class TV, class TX, class TI, bool FIRST>
 __global__ void __launch_bounds__(BLOCK) k_krylov_1t(KrylovArgs<TV, TX, TI> a,
                                                      const SolverScalars* sc) {
     using P2 = typename Pair2<TX>::type;
-    if (sc->done) return;
+    // the done flag and beta are loaded together, and the row's own pair is
+    // requested before the done test waits on them
+    const int done = sc->done;
+    const TX beta = (TX)sc->betas;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int i = lb * BLOCK + threadIdx.x;
-    const TX beta = (TX)sc->betas;
     const P2* __restrict__ rq = a.rq;
+    P2 ui{};
+    if (i < a.n) ui = rq[i];
+    if (done) return;
     auto pn = [&](int j) -> TX {
         const P2 u = rq[j];
         return FIRST ? u.x : u.x + beta * u.y;
     };
     double v = 0.0;
     if (i < a.n) {
-        const P2 ui = rq[i];
         const TX pi = FIRST ? ui.x : ui.x + beta * ui.y;
         const TX s = spmv_any(a, i, pi, pn);
         a.p[i] = pi;
@@ -102,18 +106,22 @@ template <class TV, class TX, class TI>
 __global__ void __launch_bounds__(BLOCK) k_krylov_2(KrylovArgs<TV, TX, TI> a,
                                                     const SolverScalars* sc) {
     using P2 = typename Pair2<TX>::type;
-    if (sc->done) return;
+    // loads ahead of the done test, as in k_krylov_1t
+    const int done = sc->done;
+    const TX alpha = (TX)sc->alpha;
     const int lb = xcd_block_rev(blockIdx.x, gridDim.x);  // opposite order to k_krylov_1t
     const int i = lb * BLOCK + threadIdx.x;
-    const TX alpha = (TX)sc->alpha;
     const P2* __restrict__ rv = a.rv;
+    P2 ui{};
+    if (i < a.n) ui = rv[i];
+    if (done) return;
     auto sf = [&](int j) -> TX {
         const P2 u = rv[j];
         return u.x - alpha * u.y;
     };
     double v0 = 0.0, v1 = 0.0;
     if (i < a.n) {
-        const TX si = sf(i);
+        const TX si = ui.x - alpha * ui.y;
         const TX s = spmv_any(a, i, si, sf);
         a.si[i] = si;
         a.mm2[i] = s;
@@ -132,13 +140,19 @@ template <class TV, class TX, class TI>
 __global__ void __launch_bounds__(BLOCK) k_krylov_3(KrylovArgs<TV, TX, TI> a,
                                                     const SolverScalars* sc) {
     using P2 = typename Pair2<TX>::type;
-    if (sc->done) return;
+    // loads ahead of the done test, as in k_krylov_1t
+    const int done = sc->done;
+    const TX alpha = (TX)sc->alpha, w = (TX)sc->wi;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int i = lb * BLOCK + threadIdx.x;
-    const TX alpha = (TX)sc->alpha, w = (TX)sc->wi;
+    TX si = (TX)0, p = (TX)0;
+    if (i < a.n) {
+        si = a.si[i];
+        p = a.p[i];
+    }
+    if (done) return;
     double v0 = 0.0, v1 = 0.0;
     if (i < a.n) {
-        const TX si = a.si[i], p = a.p[i];
         const TX mm1 = a.rv[i].y;
         const TX x = (a.xs[i] + alpha * p) + w * si;
         const TX r = si - w * a.mm2[i];
