@@ -202,15 +202,17 @@ template <class T>
 __global__ void k_mg_restrict1(GridInfo g, const int* __restrict__ bstart,
                                const T* __restrict__ r0, T* __restrict__ b1,
                                const SolverScalars* sc) {
-    if (sc->done) return;
+    const int done = sc->done;  // tested once the cell's bin range is requested
     const int nc = g.nx * g.ny;
     int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= nc) return;
     const int cx = c / g.ny, cy = c - cx * g.ny;
     int lo, hi;
     column_bins(g, cx, cy, cy, lo, hi);
+    const int i0 = bstart[lo], i1 = bstart[hi];
+    if (done) return;
     T s = (T)0;
-    for (int i = bstart[lo]; i < bstart[hi]; ++i) s += r0[i];
+    for (int i = i0; i < i1; ++i) s += r0[i];
     b1[c] = s;
 }
 
@@ -243,11 +245,12 @@ template <class T>
 __global__ void k_mg_down(int nx, int ny, const T* __restrict__ A,
                           const T* __restrict__ b, T* __restrict__ x, int nx2, int ny2,
                           T* __restrict__ b2, const SolverScalars* sc) {
-    if (sc->done) return;
+    const int done = sc->done;
     const int nc = nx * ny;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     const int C = t >> 2, k = t & 3;
-    T r = (T)0;
+    T r = (T)0, xc = (T)0;
+    int cw = -1;
     if (C < nx2 * ny2) {
         const int X = C / ny2, Y = C - X * ny2;
         const int cx = 2 * X + (k >> 1), cy = 2 * Y + (k & 1);
@@ -259,10 +262,14 @@ __global__ void k_mg_down(int nx, int ny, const T* __restrict__ A,
                 const int ex = mg_clamp(cx + d / 3 - 1, nx), ey = mg_clamp(cy + d % 3 - 1, ny);
                 acc += A[(size_t)d * nc + c] * mg_jac0(A, nc, ex * ny + ey, b);
             }
-            x[c] = mg_jac0(A, nc, c, b);
+            xc = mg_jac0(A, nc, c, b);
             r = b[c] - acc;
+            cw = c;
         }
     }
+    // the flag was loaded with the first operands; uniform over the grid
+    if (done) return;
+    if (cw >= 0) x[cw] = xc;
     const T s = r + __shfl_xor(r, 1);
     const T q = s + __shfl_xor(s, 2);
     if (k == 0 && C < nx2 * ny2) b2[C] = q;
@@ -379,7 +386,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
                                                       const T* __restrict__ Ainv,
                                                       const int* __restrict__ Aok,
                                                       const SolverScalars* sc) {
-    if (sc->done) return;
+    const int done = sc->done;  // tested once the prologue's loads are requested
     __shared__ T xt[2][MG_CMAX];   // top level: iterate (two parities), residual
     __shared__ T rt[MG_CMAX];
     __shared__ T As[9 * MG_CLOW];  // lower levels: stencil d of cell c at 9 off + d n + c
@@ -447,6 +454,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
         xl[0][e] = (T)0;
         xl[1][e] = (T)0;
     }
+    if (done) return;  // uniform over the block
     __syncthreads();
 
     // top sweep from parity p (z = x + e(parent) on level 1 at parity q if corr)
@@ -628,7 +636,7 @@ __global__ void k_mg_grid_up(int nx, int ny, const T* __restrict__ A,
                              const T* __restrict__ b, const T* __restrict__ x, int ny2,
                              const T* __restrict__ e, float over, T* __restrict__ out,
                              const SolverScalars* sc) {
-    if (sc->done) return;
+    const int done = sc->done;  // tested before the store
     const int nc = nx * ny;
     int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= nc) return;
@@ -643,7 +651,9 @@ __global__ void k_mg_grid_up(int nx, int ny, const T* __restrict__ A,
     }
     const T dg = A[(size_t)4 * nc + c];
     const T zc = z(cx, cy);
-    out[c] = dg != (T)0 ? zc + mg_omega<T>() * (b[c] - acc) / dg : (T)0;
+    const T v = dg != (T)0 ? zc + mg_omega<T>() * (b[c] - acc) / dg : (T)0;
+    if (done) return;
+    out[c] = v;
 }
 
 // Two grid levels in one launch.  Each dependent launch of a cell level
@@ -667,7 +677,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_down2(int nx0, int ny0, const T* _
                                                     T* __restrict__ b1, T* __restrict__ x1,
                                                     int nx2, int ny2, T* __restrict__ b2,
                                                     const SolverScalars* sc) {
-    if (sc->done) return;
+    const int done = sc->done;  // tested once the first loads are requested
     constexpr int TX = MG_TX, TY = MG_TY;
     constexpr int MW = 2 * TX + 2, MH = 2 * TY + 2;  // level 1 under the tile, halo 1
     constexpr int RW = 2 * MW, RH = 2 * MH;          // their level-0 children
@@ -686,15 +696,29 @@ __global__ void __launch_bounds__(BLOCK) k_mg_down2(int nx0, int ny0, const T* _
     const T omega = mg_omega<T>();
     // level 0 from zero, x = omega D^-1 b (each position holds its clamped
     // cell's value, as the sweeps read clamped neighbours)
-    for (int e = threadIdx.x; e < FW * FH; e += BLOCK) {
-        const int i = e / FH, j = e - i * FH;
-        const int ox = fx0 + i, oy = fy0 + j;
-        const int c = mg_clamp(ox, nx0) * ny0 + mg_clamp(oy, ny0);
-        const T v = mg_jac0(A0, nc0, c, b0);
-        xf[e] = v;
-        if (ox >= 4 * X0 && ox < 4 * (X0 + TX) && ox < nx0 && oy >= 4 * Y0 &&
-            oy < 4 * (Y0 + TY) && oy < ny0)
-            x0[c] = v;
+    constexpr int NF = (FW * FH + BLOCK - 1) / BLOCK;
+    T vf[NF];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        const int e = threadIdx.x + q * BLOCK;
+        vf[q] = (T)0;
+        if (e < FW * FH) {
+            const int i = e / FH, j = e - i * FH;
+            vf[q] = mg_jac0(A0, nc0, mg_clamp(fx0 + i, nx0) * ny0 + mg_clamp(fy0 + j, ny0), b0);
+        }
+    }
+    if (done) return;  // uniform over the grid
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        const int e = threadIdx.x + q * BLOCK;
+        if (e < FW * FH) {
+            const int i = e / FH, j = e - i * FH;
+            const int ox = fx0 + i, oy = fy0 + j;
+            xf[e] = vf[q];
+            if (ox >= 4 * X0 && ox < 4 * (X0 + TX) && ox < nx0 && oy >= 4 * Y0 &&
+                oy < 4 * (Y0 + TY) && oy < ny0)
+                x0[mg_clamp(ox, nx0) * ny0 + mg_clamp(oy, ny0)] = vf[q];
+        }
     }
     __syncthreads();
     // level-0 residuals of the children of the level-1 region (0 outside)
@@ -785,7 +809,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_up2(int nx0, int ny0, const T* __r
                                                   const T* __restrict__ e2, float over,
                                                   T* __restrict__ out1, T* __restrict__ out0,
                                                   const SolverScalars* sc) {
-    if (sc->done) return;
+    const int done = sc->done;  // tested once the first loads are requested
     constexpr int TX = MG_TX, TY = MG_TY;
     constexpr int FW = 4 * TX + 2, FH = 4 * TY + 2;  // level 0 under the tile, halo 1
     constexpr int MW = 2 * TX + 2, MH = 2 * TY + 2;  // their level-1 parents
@@ -807,6 +831,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_up2(int nx0, int ny0, const T* __r
         const int ex = mg_clamp(zx0 + i, nx1), ey = mg_clamp(zy0 + j, ny1);
         z1[e] = x1[ex * ny1 + ey] + f * e2[(ex >> 1) * ny2 + (ey >> 1)];
     }
+    if (done) return;  // uniform over the grid
     __syncthreads();
     // level-1 sweep over its region; the tile's own cells are written
     for (int e = threadIdx.x; e < MW * MH; e += BLOCK) {
