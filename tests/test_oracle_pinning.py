@@ -419,6 +419,18 @@ def test_dot_tree_accuracy():
         assert abs(pyoracle.dot_tree(a, b) - exact) <= 1e-13 * (np.abs(a * b).sum() + 1e-300)
 
 
+def test_dot_tree_sums_every_row_once():
+    """Integer products below 2^53 sum exactly in any order, so the tree must
+    return the exact sum; a dropped or repeated row (a wrong block offset,
+    a short butterfly, a partial left out) changes it."""
+    rng = np.random.default_rng(6)
+    for n in (1, 64, 255, 256, 257, 1025, 5000, 70000):
+        a = rng.integers(-1000, 1000, size=n).astype(np.float64)
+        b = rng.integers(-1000, 1000, size=n).astype(np.float64)
+        exact = float(np.sum(a.astype(np.int64) * b.astype(np.int64)))
+        assert pyoracle.dot_tree(a, b) == exact
+
+
 # ------------------------------------------- correction, viscosity, shift --
 def test_comp_vel_linear_pressure_exact_gradient():
     """Symmetric form sum V (p_i + p_j) L grad W on a lattice interior with a
