@@ -522,8 +522,20 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
         }
         __syncthreads();
     };
+    // The sweep from zero of level l (x = omega D^-1 b into parity 1) is
+    // pointwise, so the thread that restricts b into cell c of level l also
+    // sweeps it, the same expression as low_sweep's, and the level's first
+    // barrier phase goes away.  Not on the level solved exactly.
+    const int Lx = D ? D : K - 1;  // the level solved exactly
+    auto zero_sweep = [&](int l, int cc, T bcc) {
+        const int n = B.nx[l] * B.ny[l], o = B.off[l];
+        const T dg = As[9 * o + 4 * n + cc];
+        const T nv = omega * bcc / dg;
+        xl[1][o + cc] = dg != (T)0 ? nv : (T)0;
+    };
     // residual of level l at parity p, summed over the children of each cell
-    // of level l+1 into its right-hand side
+    // of level l+1 into its right-hand side (and the sweep from zero of
+    // level l+1 unless it is solved exactly)
     auto low_restrict = [&](int l, int p) {
         const int nx = B.nx[l], ny = B.ny[l], n = nx * ny, o = B.off[l];
         const int ny2 = B.ny[l + 1];
@@ -546,6 +558,7 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
                     s += bl[o + cc] - acc;
                 }
             bl[B.off[l + 1] + c] = s;
+            if (l + 1 < Lx) zero_sweep(l + 1, c, s);
         }
         __syncthreads();
     };
@@ -578,13 +591,13 @@ __global__ void __launch_bounds__(MG_CMAX) k_mg_block(MGBlock B, const T* __rest
                         if (cx < B.nx[0] && cy < ny0) s += rt[cx * ny0 + cy];
                     }
                 bl[B.off[1] + c] = s;
+                if (1 < Lx) zero_sweep(1, c, s);
             }
             __syncthreads();
         }
-        // down the lower levels: smoothing from zero, restriction
-        const int Lx = D ? D : K - 1;  // the level solved exactly
+        // down the lower levels: smoothing (the sweep from zero was made with
+        // the restriction into the level), restriction
         for (int l = 1; l < Lx; ++l) {
-            low_sweep(l, 0, true, false, 0);
             int p = 1;
             for (int s = 1; s < nu; ++s) {
                 low_sweep(l, p, false, false, 0);
