@@ -658,8 +658,10 @@ int run_solve(isph_ctx* c) {
     int rc = prec_mg(c->prm.precision) ? solve_mg(c)
              : mixed                                ? solve_mixed(c)
                                                     : solve_fp64(c);
-    int rc2 = read_sc(c);
-    if (rc == ISPH_OK) rc = rc2;
+    // every solve returns on success right after reading the solver state
+    // (a second read would cost a synchronisation); after a failure the state
+    // is read again for the counts below
+    if (rc != ISPH_OK) (void)read_sc(c);
     c->iters = c->sc_host->total;
     c->relres = c->sc_host->relres;
     c->outer = mixed ? c->sc_host->nupd : 0;
