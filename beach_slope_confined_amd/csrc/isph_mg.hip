@@ -83,8 +83,9 @@ __global__ void k_mg_cells(int n, GridInfo g, const int* __restrict__ key, int* 
 // lists built from the cell stencil).  k_mg_galerkin1: per cell, its rows'
 // sums in row order.  A thread per cell walking its rows' entries serially
 // (about 140 at 2h cells) left the GPU nearly idle: 606 blocks at 1M.
+template <class TI>
 __global__ void __launch_bounds__(BLOCK) k_mg_rows9(int n, int ny, size_t ld,
-                                                    const int* __restrict__ nbr,
+                                                    const TI* __restrict__ nbr,
                                                     const int* __restrict__ cnt,
                                                     const double* __restrict__ val,
                                                     const int* __restrict__ cell,
@@ -114,7 +115,8 @@ __global__ void __launch_bounds__(BLOCK) k_mg_rows9(int n, int ny, size_t ld,
     // entry order
     for (; k + 4 <= c; k += 4) {
         const size_t o = (size_t)k * ld + i;
-        const int j0 = nbr[o], j1 = nbr[o + ld], j2 = nbr[o + 2 * ld], j3 = nbr[o + 3 * ld];
+        const int j0 = mg_col(nbr[o], i), j1 = mg_col(nbr[o + ld], i);
+        const int j2 = mg_col(nbr[o + 2 * ld], i), j3 = mg_col(nbr[o + 3 * ld], i);
         const double v0 = val[o], v1 = val[o + ld], v2 = val[o + 2 * ld], v3 = val[o + 3 * ld];
         const int c0 = cell[j0], c1 = cell[j1], c2 = cell[j2], c3 = cell[j3];
         add(c0, v0);
@@ -124,7 +126,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_rows9(int n, int ny, size_t ld,
     }
     for (; k < c; ++k) {
         const size_t o = (size_t)k * ld + i;
-        add(cell[nbr[o]], val[o]);
+        add(cell[mg_col(nbr[o], i)], val[o]);
     }
     if (bad) atomicOr(err, ERR_DOMAIN);
 #pragma unroll
@@ -144,7 +146,20 @@ __global__ void k_mg_galerkin1(GridInfo g, const int* __restrict__ bstart, size_
     int lo, hi;
     column_bins(g, cx, cy, cy, lo, hi);
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = bstart[lo]; i < bstart[hi]; ++i)
+    const int i1 = bstart[hi];
+    int i = bstart[lo];
+    // two rows' sums loaded before they are added, in row order per plane
+    for (; i + 2 <= i1; i += 2) {
+        double r0[9], r1[9];
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            r0[d] = part9[(size_t)d * ld + i];
+            r1[d] = part9[(size_t)d * ld + i + 1];
+        }
+#pragma unroll
+        for (int d = 0; d < 9; ++d) acc[d] = (acc[d] + r0[d]) + r1[d];
+    }
+    if (i < i1)
 #pragma unroll
         for (int d = 0; d < 9; ++d) acc[d] += part9[(size_t)d * ld + i];
 #pragma unroll
@@ -1231,19 +1246,17 @@ static int mg_alloc(isph_ctx* c) {
 }
 
 // Galerkin hierarchy of the system's matrix (FP64 sums, stored as T)
-template <class T>
-static int mg_setup(isph_ctx* c, const MGSystem& sys) {
+template <class T, class TI>
+static int mg_setup(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
     int rc = mg_alloc(c);
     if (rc) return rc;
     MGData* m = c->mg;
     const GridInfo& g = c->grid;
     k_mg_cells<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, g, c->key_s, m->cell);
     if (c->n)
-        k_mg_rows9<<<nblk(c->n), BLOCK, 0, c->stream>>>(c->n, g.ny, (size_t)c->cap, c->nbr, c->cnt,
-                                                        sys.val64, m->cell, m->part9,
-                                                        std::is_same<T, float>::value ? m->p9f
-                                                                                      : nullptr,
-                                                        c->err);
+        k_mg_rows9<TI><<<nblk(c->n), BLOCK, 0, c->stream>>>(
+            c->n, g.ny, (size_t)c->cap, nbr, c->cnt, sys.val64, m->cell, m->part9,
+            std::is_same<T, float>::value ? m->p9f : nullptr, c->err);
     k_mg_galerkin1<T><<<nblk((int64_t)g.nx * g.ny), BLOCK, 0, c->stream>>>(
         g, c->bincnt, (size_t)c->cap, m->part9, as<T>(m->lv[0].A));
     // the grid levels' operators on the main stream, the block levels'
@@ -1381,7 +1394,7 @@ static int solve_mg_body(isph_ctx* c, const MGSystem& sys, const TI* nbr) {
     const int grid = nblk8(c->n);
     double* x = sys.x;
     double* r64 = c->v64[9];
-    int rc = mg_setup<T>(c, sys);
+    int rc = mg_setup<T, TI>(c, sys, nbr);
     if (rc) return rc;
     SolveCtl ctl = solve_ctl(c, 1);
     SolveCtl ctl_flush = solve_ctl(c, 2);
