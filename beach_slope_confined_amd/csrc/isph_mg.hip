@@ -692,7 +692,7 @@ __global__ void k_mg_grid_up(int nx, int ny, const T* __restrict__ A,
 // order, with the same clamped neighbours as k_mg_down / k_mg_grid_up, so
 // the results are theirs bit for bit; each cell is written by the one tile
 // that owns it.
-constexpr int MG_TX = 4, MG_TY = 8;
+constexpr int MG_TX = 2, MG_TY = 8;
 
 // k_mg_down for levels 0 and 1 of the pair: level 0 (nx0 x ny0) smoothed
 // from zero and restricted into level 1, level 1 likewise into level 2.
