@@ -34,8 +34,11 @@
 // ISPH_PREC_FP64_MG they are FP64 (kernels templated on the value type T).
 //
 // The hierarchy is rebuilt for every solve (the matrix changes every step):
-// two kernels per level, one thread per coarse cell summing its children in
-// a fixed order, so the setup is deterministic.
+// each row's entries summed by neighbour cell offset (k_mg_rows9), the cell
+// operator from those sums (k_mg_galerkin1), then one launch per coarser
+// level (k_mg_galerkin), each sum in a fixed order, so the set-up is
+// deterministic.  The block levels' operators and the exact level's
+// inverse are formed on a side stream (mg_setup, mg_join).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
